@@ -1,0 +1,113 @@
+! mglc_lbm_capi.f90 -- iso_c_binding interfaces to libmglc_lbm.so
+! (include/mglc_lbm.h).  Each interface replaces one of the reference's
+! no-argument subroutines of MPI/Lid_driven_cavity/fortran/3d/mpi_3d_blocked/:
+!   lbm_initial        initial.f90:1-76
+!   lbm_step(n)        n x (collision.f90 + streaming.f90 + bounce_back.f90 + macro.f90)
+!   lbm_check          check.f90:9-34 (returns errorU)
+!   lbm_download_macro the gather half of output.f90:1-133 (one GPU = one rank)
+module mglc_lbm_capi
+    use iso_c_binding
+    implicit none
+
+    integer(c_int), parameter :: LBM_OK = 0, LBM_ERR_ARG = 1, LBM_ERR_HIP = 2, LBM_ERR_NAN = 3
+    integer(c_int), parameter :: LBM_ERR_TAU = 4, LBM_ERR_NOMEM = 5, LBM_ERR_STATE = 6
+    integer(c_int), parameter :: LBM_MRT = 0, LBM_BGK = 1
+    integer(c_int), parameter :: LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16 = 2
+
+    ! mirrors struct lbm_params
+    type, bind(C) :: lbm_params
+        integer(c_int) :: nx, ny, nz
+        integer(c_int) :: collision
+        integer(c_int) :: storage
+        integer(c_int) :: meq12_rho
+        integer(c_int) :: f16_scale_log2
+        integer(c_int) :: thermal
+        integer(c_int) :: block_x, block_y
+        integer(c_int) :: reserved_i(6)
+        real(c_double) :: tau
+        real(c_double) :: u_lid
+        real(c_double) :: rho0
+        real(c_double) :: reserved_d(8)
+    end type lbm_params
+
+    interface
+        integer(c_int) function lbm_create(p, ctx) bind(C, name='lbm_create')
+            import :: c_int, c_ptr, lbm_params
+            type(lbm_params), intent(in) :: p
+            type(c_ptr), intent(out) :: ctx
+        end function lbm_create
+
+        integer(c_int) function lbm_destroy(ctx) bind(C, name='lbm_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function lbm_destroy
+
+        integer(c_int) function lbm_initial(ctx) bind(C, name='lbm_initial')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function lbm_initial
+
+        integer(c_int) function lbm_load_fields(ctx, rho, u, v, w) bind(C, name='lbm_load_fields')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: rho(*), u(*), v(*), w(*)
+        end function lbm_load_fields
+
+        integer(c_int) function lbm_step(ctx, nsteps) bind(C, name='lbm_step')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsteps
+        end function lbm_step
+
+        integer(c_int) function lbm_check(ctx, error_u) bind(C, name='lbm_check')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: error_u
+        end function lbm_check
+
+        integer(c_int) function lbm_download_macro(ctx, rho, u, v, w) bind(C, name='lbm_download_macro')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: rho(*), u(*), v(*), w(*)
+        end function lbm_download_macro
+
+        integer(c_int) function lbm_synchronize(ctx) bind(C, name='lbm_synchronize')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function lbm_synchronize
+
+        type(c_ptr) function lbm_last_error_c() bind(C, name='lbm_last_error')
+            import :: c_ptr
+        end function lbm_last_error_c
+    end interface
+
+contains
+
+    ! The message of the last failed call, as a Fortran string.
+    function lbm_last_error() result(msg)
+        character(len=512) :: msg
+        character(kind=c_char), pointer :: s(:)
+        type(c_ptr) :: p
+        integer :: i
+        msg = ' '
+        p = lbm_last_error_c()
+        if (.not. c_associated(p)) return
+        call c_f_pointer(p, s, [512])
+        do i = 1, 512
+            if (s(i) == c_null_char) exit
+            msg(i:i) = s(i)
+        enddo
+    end function lbm_last_error
+
+    ! Stop with the status, in the reference's style of fatal errors
+    ! (stop on failure; the reference's MPI codes call MPI_Abort).
+    subroutine lbm_require(rc, where)
+        integer(c_int), intent(in) :: rc
+        character(len=*), intent(in) :: where
+        if (rc /= LBM_OK) then
+            write(*,*) 'error: ', where, ' returned ', rc, ': ', trim(lbm_last_error())
+            error stop 2
+        endif
+    end subroutine lbm_require
+
+end module mglc_lbm_capi

@@ -1,0 +1,204 @@
+"""Python binding of libmglc_lbm.so (the C-ABI of include/mglc_lbm.h).
+
+The reference drives its step as Fortran subroutines on module globals
+(MPI/Lid_driven_cavity/fortran/3d/mpi_3d_blocked/main.f90:75-103):
+``initial``, then ``collision / streaming / bounceback / macro`` per step,
+``check`` every 2000 steps and ``output`` at the end.  :class:`Cavity` exposes
+the same operations -- ``initial()``, ``step(n)``, ``check()``,
+``macro()`` -- backed only by the HIP library.  There is no CPU fallback: if
+the library is missing or fails to load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmglc_lbm.so")
+
+LBM_MRT, LBM_BGK = 0, 1
+STORE_F64, STORE_F32, STORE_F16 = 0, 1, 2
+STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16}
+COLLISION_NAMES = {"mrt": LBM_MRT, "bgk": LBM_BGK}
+
+LBM_OK, LBM_ERR_NAN = 0, 3
+
+# Every entry point include/mglc_lbm.h declares.
+EXPORTED_SYMBOLS = (
+    "lbm_create", "lbm_destroy", "lbm_set_stream", "lbm_initial", "lbm_load_fields", "lbm_step",
+    "lbm_check", "lbm_download_macro", "lbm_synchronize", "lbm_get_info", "lbm_status_string",
+    "lbm_last_error",
+)
+
+
+class LbmParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("collision", ctypes.c_int), ("storage", ctypes.c_int), ("meq12_rho", ctypes.c_int),
+        ("f16_scale_log2", ctypes.c_int), ("thermal", ctypes.c_int),
+        ("block_x", ctypes.c_int), ("block_y", ctypes.c_int),
+        ("reserved_i", ctypes.c_int * 6),
+        ("tau", ctypes.c_double), ("u_lid", ctypes.c_double), ("rho0", ctypes.c_double),
+        ("reserved_d", ctypes.c_double * 8),
+    ]
+
+
+class LbmInfo(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_longlong), ("ny", ctypes.c_longlong), ("nz", ctypes.c_longlong),
+        ("pitch_x", ctypes.c_longlong), ("pop_stride", ctypes.c_longlong),
+        ("steps_done", ctypes.c_longlong), ("device_bytes", ctypes.c_longlong),
+        ("bytes_per_lup", ctypes.c_double), ("storage", ctypes.c_int), ("collision", ctypes.c_int),
+    ]
+
+
+class LbmError(RuntimeError):
+    def __init__(self, status: int, where: str, message: str):
+        super().__init__(f"{where}: status {status}: {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the HIP library.  Raises if it is not built -- no fallback."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libmglc_lbm.so is not built ({path}); run __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int
+    lib.lbm_create.argtypes = [ctypes.POINTER(LbmParams), ctypes.POINTER(vp)]
+    lib.lbm_destroy.argtypes = [vp]
+    lib.lbm_set_stream.argtypes = [vp, vp]
+    lib.lbm_initial.argtypes = [vp]
+    lib.lbm_load_fields.argtypes = [vp, dp, dp, dp, dp]
+    lib.lbm_step.argtypes = [vp, ip]
+    lib.lbm_check.argtypes = [vp, dp]
+    lib.lbm_download_macro.argtypes = [vp, dp, dp, dp, dp]
+    lib.lbm_synchronize.argtypes = [vp]
+    lib.lbm_get_info.argtypes = [vp, ctypes.POINTER(LbmInfo)]
+    lib.lbm_status_string.argtypes = [ip]
+    lib.lbm_status_string.restype = ctypes.c_char_p
+    lib.lbm_last_error.restype = ctypes.c_char_p
+    for name in EXPORTED_SYMBOLS:
+        if name not in ("lbm_status_string", "lbm_last_error"):
+            getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+@dataclass
+class CavityConfig:
+    """Parameters named as in the reference's module commondata
+    (mpi_3d_blocked/commondata.f90:4-42)."""
+    nx: int
+    ny: int
+    nz: int
+    Reynolds: float = 1000.0
+    U0: float = 0.1
+    rho0: float = 1.0
+    collision: str = "mrt"
+    storage: str = "f64"
+    tauf: float | None = None      # default: commondata.f90:9, 3 U0 N / Re + 0.5 with N = nx
+    meq12_rho: int = 0
+    f16_scale_log2: int = 0
+    block_x: int = 0
+    block_y: int = 0
+
+    def tau(self) -> float:
+        if self.tauf is not None:
+            return float(self.tauf)
+        return self.U0 * float(self.nx) / self.Reynolds * 3.0 + 0.5
+
+    def params(self) -> LbmParams:
+        p = LbmParams()
+        p.nx, p.ny, p.nz = self.nx, self.ny, self.nz
+        p.collision = COLLISION_NAMES[self.collision]
+        p.storage = STORAGE_NAMES[self.storage]
+        p.meq12_rho = self.meq12_rho
+        p.f16_scale_log2 = self.f16_scale_log2
+        p.block_x, p.block_y = self.block_x, self.block_y
+        p.tau, p.u_lid, p.rho0 = self.tau(), self.U0, self.rho0
+        return p
+
+
+class Cavity:
+    """One GPU's lid-driven cavity.  Arrays are returned as (nz, ny, nx) C-order
+    numpy arrays, i.e. the reference's Fortran (nx, ny, nz) layout."""
+
+    def __init__(self, cfg: CavityConfig, stream: int | None = None):
+        self.cfg = cfg
+        self.lib = load_library()
+        self._ctx = ctypes.c_void_p()
+        p = cfg.params()
+        self._call("lbm_create", ctypes.byref(p), ctypes.byref(self._ctx))
+        if stream is not None:
+            self._call("lbm_set_stream", self._ctx, ctypes.c_void_p(stream))
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, name)(*args)
+        if rc != LBM_OK:
+            raise LbmError(rc, name, self.lib.lbm_last_error().decode())
+        return rc
+
+    @property
+    def shape(self):
+        return (self.cfg.nz, self.cfg.ny, self.cfg.nx)
+
+    def initial(self):
+        self._call("lbm_initial", self._ctx)
+
+    def load_fields(self, rho, u, v, w):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(self.shape) for a in (rho, u, v, w)]
+        self._call("lbm_load_fields", self._ctx, *[_dp(a) for a in arrs])
+
+    def step(self, nsteps: int = 1):
+        self._call("lbm_step", self._ctx, int(nsteps))
+
+    def check(self) -> float:
+        e = ctypes.c_double()
+        self._call("lbm_check", self._ctx, ctypes.byref(e))
+        return e.value
+
+    def macro(self):
+        out = [np.empty(self.shape, dtype=np.float64) for _ in range(4)]
+        self._call("lbm_download_macro", self._ctx, *[_dp(a) for a in out])
+        return tuple(out)
+
+    def synchronize(self):
+        self._call("lbm_synchronize", self._ctx)
+
+    def info(self) -> LbmInfo:
+        inf = LbmInfo()
+        self._call("lbm_get_info", self._ctx, ctypes.byref(inf))
+        return inf
+
+    def close(self):
+        if self._ctx:
+            self.lib.lbm_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

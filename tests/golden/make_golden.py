@@ -1,0 +1,51 @@
+"""Generate the golden fixtures of the D3Q19 cavity from the reference itself.
+
+Runs oracle/_ref/ref_cavity -- the reference's own Fortran subroutines
+(MPI/Lid_driven_cavity/fortran/3d/mpi_3d_blocked/{commondata,initial,
+collision,streaming,bounce_back,macro}.f90), compiled unmodified by
+oracle/ref/Makefile -- and stores rho, u, v, w after n steps.  Runs only in a
+container that has /root/reference; the fixtures are committed.
+
+The reference's tauf, U0 and Reynolds are compile-time parameters
+(commondata.f90:4-9: Re = 1000, U0 = 0.1, N = 65, tauf = 0.5195), so the
+fixtures are MRT at that tau on small grids of varying shape.
+
+usage: python tests/golden/make_golden.py
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from oracle import oracle as O  # noqa: E402
+
+# (nx, ny, nz, steps): shapes are non-cubic so that a transposed axis fails.
+CASES = [
+    (8, 9, 10, 1),
+    (8, 9, 10, 2),
+    (8, 9, 10, 10),
+    (8, 9, 10, 100),
+    (12, 10, 9, 300),
+    (16, 16, 16, 500),
+    (20, 17, 13, 1000),
+]
+
+
+def main():
+    subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle", "ref", "Makefile")], check=True)
+    with tempfile.TemporaryDirectory() as td:
+        for nx, ny, nz, ns in CASES:
+            hdr, rho, u, v, w = O.run_reference_cavity(nx, ny, nz, ns, os.path.join(td, "out.bin"))
+            name = os.path.join(HERE, f"cavity_mrt_{nx}x{ny}x{nz}_s{ns}.npz")
+            np.savez_compressed(name, nx=nx, ny=ny, nz=nz, steps=ns, tauf=hdr["tauf"], U0=hdr["U0"],
+                                rho0=hdr["rho0"], rho=rho, u=u, v=v, w=w)
+            print("wrote", os.path.relpath(name, ROOT))
+
+
+if __name__ == "__main__":
+    main()

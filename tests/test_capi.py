@@ -1,0 +1,71 @@
+"""CPU tests of the C-ABI library: it loads, exports every entry point the
+header declares, and rejects bad parameters before touching the GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mglc_lbm.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(lbm_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_the_binding_list():
+    from mglc_amd import lbm
+    assert declared_functions() == sorted(lbm.EXPORTED_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol(hip_lib):
+    for name in declared_functions():
+        assert hasattr(hip_lib, name), name
+
+
+def test_create_rejects_bad_tau_and_grid(hip_lib):
+    from mglc_amd import lbm
+    ctx = ctypes.c_void_p()
+    p = lbm.CavityConfig(8, 8, 8, tauf=0.5).params()
+    assert hip_lib.lbm_create(ctypes.byref(p), ctypes.byref(ctx)) == 4  # LBM_ERR_TAU
+    assert b"tau" in hip_lib.lbm_last_error()
+    p = lbm.CavityConfig(1, 8, 8).params()
+    assert hip_lib.lbm_create(ctypes.byref(p), ctypes.byref(ctx)) == 1  # LBM_ERR_ARG
+    p = lbm.CavityConfig(8, 8, 8, block_x=48).params()
+    assert hip_lib.lbm_create(ctypes.byref(p), ctypes.byref(ctx)) == 1
+    assert not ctx.value
+    assert hip_lib.lbm_step(None, 1) == 1
+    assert hip_lib.lbm_status_string(3) == b"residual is NaN"
+
+
+def test_struct_layout_matches_header():
+    """ctypes mirror of lbm_params / lbm_info has the C sizes (ints then doubles)."""
+    from mglc_amd import lbm
+    assert ctypes.sizeof(lbm.LbmParams) == 16 * 4 + 11 * 8
+    assert ctypes.sizeof(lbm.LbmInfo) == 7 * 8 + 8 + 2 * 4
+
+
+def _driver():
+    from mglc_amd import build as B
+    exe = os.path.join(B.BIN_DIR, "mglc_cavity")
+    if not os.path.exists(exe):
+        B.build_fortran_driver()
+    return exe
+
+
+def test_fortran_driver_abi_matches():
+    """The Fortran bind(C) mirror of lbm_params has the C struct's size."""
+    import subprocess
+    from mglc_amd import lbm
+    out = subprocess.run([_driver(), "--abi"], capture_output=True, text=True, check=True).stdout
+    assert int(out.split()[0]) == ctypes.sizeof(lbm.LbmParams)
+
+
+def test_fortran_driver_binds_only_declared_symbols():
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--undefined-only", _driver()], capture_output=True, text=True,
+                         check=True).stdout
+    used = sorted({l.split()[-1] for l in out.splitlines() if " lbm_" in l})
+    assert used and set(used) <= set(declared_functions())
