@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--variants", default="f32,f16", help="comma list of reduced storages to run ('' = none)")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle on a bounded sample")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device copies of this many MiB (counter calibration) and exit")
     return ap.parse_args()
 
 
@@ -109,6 +111,11 @@ def main():
         print(f"bench.py: --gpus {args.gpus} refused: this build runs the cavity on one GPU "
               "(the multi-GPU halo path is design only)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        from mglc_amd.hip import device_copy_calibration
+        device_copy_calibration(args.copy_calibration << 20, 5)
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration << 20, "reps": 5}))
+        return
     from mglc_amd.hip import Stream
 
     stream = Stream()

@@ -45,6 +45,19 @@ enum lbm_storage { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16 = 2 };
 
 typedef struct lbm_ctx lbm_ctx;
 
+/* Halo regions of a subdomain (mpi_3d_blocked/main.f90:50-72, ex_sendrecv.f90:12-123):
+ * six faces carry the 5 populations that cross them over the whole face plane,
+ * twelve edges carry the one diagonal population that crosses both faces along
+ * the edge line; corners carry nothing in D3Q19.  Edge regions are numbered by
+ * that population a = 7..18 as region 6 + (a - 7).  The outgoing region r of a
+ * subdomain goes to the neighbour across r; what arrives through region r
+ * comes from that neighbour's outgoing region opposite to r (faces) or its
+ * outgoing region r (edges: the same diagonal population). */
+enum lbm_region {
+    LBM_FACE_XM = 0, LBM_FACE_XP = 1, LBM_FACE_YM = 2, LBM_FACE_YP = 3,
+    LBM_FACE_ZM = 4, LBM_FACE_ZP = 5, LBM_EDGE_FIRST = 6, LBM_NUM_REGIONS = 18
+};
+
 /* Run parameters.  Field names follow the reference's module commondata
  * (mpi_3d_blocked/commondata.f90:4-42, buoyancy module.F90:11-57); derived
  * constants (tauf, paraA, gBeta, ...) are computed by the caller, as the
@@ -56,13 +69,31 @@ typedef struct lbm_params {
     int meq12_rho;        /* 0: cavity meq(12) = -(v^2-w^2)/2, no rho (collision.f90:85);
                              1: thermal form with rho (evolution_f.F90:75) */
     int f16_scale_log2;   /* FP16 storage holds (f - w) * 2^k; 0 selects k = 12 */
-    int thermal;          /* 0 only in this version (D3Q7 coupling is a later row) */
-    int block_x, block_y; /* launch shape; 0 = default 64 x 4 */
-    int reserved_i[6];
-    double tau;           /* tauf: commondata.f90:9 tauf = 3 U0 N / Re + 0.5 */
-    double u_lid;         /* U0: lid velocity along +x on the top wall k = nz */
+    int thermal;          /* 1: buoyancy-driven cavity -- D3Q19 MRT with the Guo body force
+                             (Boussinesq + Coriolis) coupled to a D3Q7 MRT temperature
+                             (buoyancy .../3d/mpi_blocked/evolution_f.F90, evolution_g.F90);
+                             FP64 storage, all walls no-slip, no lid */
+    int block_x, block_y; /* launch shape; 0 = default (x: min(256, nx rounded to 64)) */
+    int halo_faces;       /* bit f (enum lbm_region, faces 0..5) set: face f borders a
+                             neighbour subdomain and reads its halo; clear: wall
+                             (halfway bounce-back; the z+ wall is the moving lid) */
+    int halo_loopback;    /* 1: before every step, pack all regions and unpack each
+                             into the opposite region of this same subdomain (the
+                             exchange looped back on one GPU) */
+    int thermal_axis;     /* thermal: axis of the walls held at t_hot (low side) and t_cold
+                             (high side): 1 = y (LeftRightWallsConstT), 2 = z
+                             (TopBottomPlatesConstT), 0 = x, -1 = none; other walls adiabatic */
+    int reserved_i[3];
+    double tau;           /* tauf: commondata.f90:9 tauf = 3 U0 N / Re + 0.5
+                             (thermal: module.F90:19 tauf = 0.5 + Ma N sqrt(3 Pr / Ra)) */
+    double u_lid;         /* U0: lid velocity along +x on the top wall k = nz (0 when thermal) */
     double rho0;          /* initial density */
-    double reserved_d[8];
+    /* thermal constants, computed by the caller as module.F90:19-33 does */
+    double para_a;        /* paraA = 42 sqrt(3) kappa - 6 */
+    double g_beta;        /* gBeta: the buoyancy force is rho gBeta (T - t_ref) along +z */
+    double omega_rot;     /* rotation rate of the Coriolis force (0 = none) */
+    double t_hot, t_cold, t_ref;
+    double reserved_d[2];
 } lbm_params;
 
 typedef struct lbm_info {
@@ -84,19 +115,26 @@ int lbm_destroy(lbm_ctx *ctx);
 int lbm_set_stream(lbm_ctx *ctx, void *hip_stream);
 
 /* Reference initial state (initial.f90:46-73): rho = rho0, u = v = w = 0,
- * u = U0 on the top fluid layer, f = feq(rho, u); up = vp = wp = 0; itc = 0. */
+ * u = U0 on the top fluid layer when the z+ face is the lid wall (the
+ * reference's coords(2) == dims(2) - 1 test), f = feq(rho, u); up = vp = 0;
+ * itc = 0.  Thermal: T = 0 except the layers next to the held walls, which
+ * start at t_hot / t_cold (initial.F90:126-162). */
 int lbm_initial(lbm_ctx *ctx);
 
 /* Equilibrium start from caller fields (each nx*ny*nz FP64, x fastest):
  * f = feq(rho, u, v, w), macro fields = the given ones, itc = 0.  The hook
- * for seeded-noise starts and restarts. */
+ * for seeded-noise starts and restarts.  T is the temperature of a thermal
+ * context (g = its equilibrium, initial.F90:176-179; NULL = 0) and is
+ * ignored otherwise. */
 int lbm_load_fields(lbm_ctx *ctx, const double *rho, const double *u, const double *v,
-                    const double *w);
+                    const double *w, const double *T);
 
 /* nsteps iterations of collision -> streaming -> bounceback -> macro
  * (main.f90:87-97), fused into one pass over the lattice per step.  The
  * macroscopic fields of the last step are kept for lbm_check and
- * lbm_download_macro. */
+ * lbm_download_macro.  With halo_loopback the exchange runs before every
+ * step; with halo faces and no loopback the caller exchanges between steps
+ * (lbm_halo_pack / transport / lbm_halo_unpack), so nsteps must be 1. */
 int lbm_step(lbm_ctx *ctx, int nsteps);
 
 /* Reference check (check.f90:9-34): errorU = sqrt(sum (u-up)^2 + (v-vp)^2) /
@@ -104,6 +142,44 @@ int lbm_step(lbm_ctx *ctx, int nsteps);
  * FP64 device reduction.  Returns LBM_ERR_NAN (with *error_u set) when the
  * residual is NaN. */
 int lbm_check(lbm_ctx *ctx, double *error_u);
+
+/* Thermal residuals (buoyancy check.F90:16-41): errorU with the w term,
+ * errorT = sum |T - Tp| / sum |T|; then up, vp, wp, Tp = u, v, w, T. */
+int lbm_check_thermal(lbm_ctx *ctx, double *error_u, double *error_t);
+
+/* Nusselt numbers across the held walls of a thermal context (FP64):
+ * nu_vol  = <u_a T> N_a / kappa + 1, the convective form of RaNu.F90:11-24
+ *           with u_a the velocity along the held axis;
+ * nu_wall = mean over the hot wall of 2 (t_hot - T_first_layer) N_a / (t_hot - t_cold),
+ *           the conductive flux at the half-link wall.  kappa = (paraA + 6) / (42 sqrt 3). */
+int lbm_nusselt(lbm_ctx *ctx, double *nu_vol, double *nu_wall);
+
+/* Copy T of the last step (thermal contexts) to a host array of nx*ny*nz. */
+int lbm_download_temperature(lbm_ctx *ctx, double *T);
+
+/* The two global sums behind errorU (check.f90:12-17 before the Allreduce of
+ * check.f90:25-28): e1 = sum (u-up)^2 + (v-vp)^2, e2 = sum u^2 + v^2 + w^2 over
+ * this subdomain, then up = u, vp = v.  A decomposed run adds them over all
+ * subdomains and takes errorU = sqrt(e1) / sqrt(e2). */
+int lbm_check_sums(lbm_ctx *ctx, double *e1, double *e2);
+
+/* Packed halo buffer: all 18 regions concatenated in region order, elements of
+ * the storage type (8, 4 or 2 bytes).  Faces: [slot 0..4][plane], slots the 5
+ * populations in ascending order, plane index j + ny*k (x faces), i + nx*k
+ * (y faces), i + nx*j (z faces).  Edges: the line index along the free axis. */
+int lbm_halo_layout(const lbm_ctx *ctx, int region, long long *offset, long long *count);
+int lbm_halo_buffer_bytes(const lbm_ctx *ctx, size_t *bytes);
+
+/* Pack every outgoing region of the current state into the device buffer
+ * dev_send (lbm_halo_buffer_bytes long): the post-collision populations that
+ * leave through each face and edge, as the exchange of ex_sendrecv.f90 sends
+ * them.  Call after lbm_initial / lbm_step and before the next lbm_step. */
+int lbm_halo_pack(lbm_ctx *ctx, void *dev_send);
+
+/* Take the incoming regions from the device buffer dev_recv (same layout;
+ * region r holds what arrives through r).  Only regions of faces in
+ * halo_faces (and edges between two such faces) are read by the next step. */
+int lbm_halo_unpack(lbm_ctx *ctx, const void *dev_recv);
 
 /* Copy rho, u, v, w of the last step to host arrays (nx*ny*nz each, x
  * fastest; any pointer may be NULL to skip that field). */

@@ -26,30 +26,13 @@
 #include <cstring>
 #include <new>
 
-#include "../../include/mglc_lbm.h"
 #include "collide.hpp"
+#include "context.hpp"
+#include "halo.hpp"
 
 using namespace lbm;
 
 namespace {
-
-thread_local char g_err[512] = "";
-
-int fail(int code, const char* fmt, ...) {
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(g_err, sizeof g_err, fmt, ap);
-    va_end(ap);
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                   \
-    do {                                                                                \
-        hipError_t e_ = (expr);                                                         \
-        if (e_ != hipSuccess)                                                           \
-            return fail(LBM_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                            \
-    } while (0)
 
 struct StepArgs {
     const void* src;
@@ -67,6 +50,10 @@ struct StepArgs {
     double* u;
     double* v;
     double* w;
+    int halo_faces;     // bit f: face f reads the halo instead of bouncing back
+    const void* halo;   // incoming regions, packed layout (HALO kernels only)
+    long long hoff[NREGION];    // element offset of each region in the packed layout
+    long long hcount[NREGION];  // elements in each region
 };
 
 enum { COLL_MRT = 0, COLL_BGK = 1 };
@@ -82,10 +69,68 @@ __device__ __forceinline__ void collide(typename S::Real f[Q],
         collide_mrt<R, S::SHIFT>(f, q, R(a.snu), R(a.sq), a.meq12_rho != 0);
 }
 
-// One lattice update per thread.  Block = (64, by): one wavefront per x-row
-// segment of 64 cells, so every population load and store of a wave is 64
-// consecutive elements.
-template <class S, int COLL, bool MACRO>
+// Value pulled from outside the domain for population al at cell (i, j, k):
+// from the halo when every face the pull crosses borders a neighbour
+// subdomain, else halfway bounce-back from this cell's opposite population
+// (f(a, x) = f_post(abar, x), bounce_back.f90:6-83), with the moving-lid term
+// when the pull crosses the top wall.  A wall crossed anywhere wins, as the
+// reference's bounceback overwrites what streaming pulled from its halo.
+template <class S, int al, bool HALO>
+__device__ __forceinline__ typename S::Real pull_outside(const StepArgs& a, const typename S::T* src,
+                                                         int i, int j, int k, int c, bool top,
+                                                         double rho_lid) {
+    using T = typename S::T;
+    using R = typename S::Real;
+    constexpr int ex = EX[al], ey = EY[al], ez = EZ[al];
+    bool lid = top;  // the z+ face is the moving lid when it is a wall
+    if constexpr (HALO) {
+        bool cx = false, cy = false, cz = false;
+        if constexpr (ex != 0) cx = (unsigned)(i - ex) >= (unsigned)a.nx;
+        if constexpr (ey != 0) cy = (unsigned)(j - ey) >= (unsigned)a.ny;
+        if constexpr (ez != 0) cz = (unsigned)(k - ez) >= (unsigned)a.nz;
+        constexpr int fx = ex != 0 ? entry_face(al, 0) : 0;
+        constexpr int fy = ey != 0 ? entry_face(al, 1) : 0;
+        constexpr int fz = ez != 0 ? entry_face(al, 2) : 0;
+        const bool wall = (cx && !((a.halo_faces >> fx) & 1)) || (cy && !((a.halo_faces >> fy) & 1)) ||
+                          (cz && !((a.halo_faces >> fz) & 1));
+        if (!wall) {
+            const T* h = static_cast<const T*>(a.halo);
+            long long idx;
+            if (int(cx) + int(cy) + int(cz) == 2) {
+                constexpr int free_axis = al >= 7 ? edge_free_axis(al) : 0;
+                const int t = free_axis == 0 ? i : (free_axis == 1 ? j : k);
+                idx = a.hoff[al >= 7 ? edge_region(al) : 0] + t;
+            } else if (cx) {
+                idx = a.hoff[fx] + (long long)cross_slot(fx, al, -1) * a.ny * a.nz + (j - ey) +
+                      (long long)a.ny * (k - ez);
+            } else if (cy) {
+                idx = a.hoff[fy] + (long long)cross_slot(fy, al, -1) * a.nx * a.nz + (i - ex) +
+                      (long long)a.nx * (k - ez);
+            } else {
+                idx = a.hoff[fz] + (long long)cross_slot(fz, al, -1) * a.nx * a.ny + (i - ex) +
+                      (long long)a.nx * (j - ey);
+            }
+            return S::ld(h + idx, a.inv_scale);
+        }
+        lid = top && !((a.halo_faces >> LBM_FACE_ZP) & 1);
+    }
+    R val = S::ld(src + (long long)OPP[al] * a.ps + c, a.inv_scale);
+    if constexpr (ez == -1 && ex != 0) {
+        // moving lid (bounce_back.f90:77-78): f(14) = f_post(11) - rho/6 U0,
+        // f(13) = f_post(12) - rho/6 (-U0); the lid is applied last, so it
+        // wins over the side walls on these two populations.
+        if (lid) {
+            const double t = rho_lid / 6.0 * a.u_lid;
+            val = (ex > 0) ? R(double(val) + t) : R(double(val) - t);
+        }
+    }
+    return val;
+}
+
+// One lattice update per thread.  Block = (bx, by) with bx a multiple of 64:
+// one wavefront per x-row segment of 64 cells, so every population load and
+// store of a wave is 64 consecutive elements.
+template <class S, int COLL, bool MACRO, bool HALO>
 __global__ __launch_bounds__(256) void k_cavity_step(const StepArgs a) {
     using T = typename S::T;
     using R = typename S::Real;
@@ -117,18 +162,7 @@ __global__ __launch_bounds__(256) void k_cavity_step(const StepArgs a) {
                 const long long off = (long long)ex + (long long)ey * a.px + (long long)ez * a.pxy;
                 f[al] = S::ld(base + (c - off), a.inv_scale);
             } else {
-                // halfway bounce-back: f(a, x) = f_post(abar, x)
-                R val = S::ld(src + (long long)OPP[al] * a.ps + c, a.inv_scale);
-                if constexpr (ez == -1 && ex != 0) {
-                    // moving lid (bounce_back.f90:77-78): f(14) = f_post(11) - rho/6 U0,
-                    // f(13) = f_post(12) - rho/6 (-U0); the lid is applied last, so it
-                    // wins over the side walls on these two populations.
-                    if (top) {
-                        const double t = rho_lid / 6.0 * a.u_lid;
-                        val = (ex > 0) ? R(double(val) + t) : R(double(val) - t);
-                    }
-                }
-                f[al] = val;
+                f[al] = pull_outside<S, al, HALO>(a, src, i, j, k, c, top, rho_lid);
             }
         }
     });
@@ -147,6 +181,64 @@ __global__ __launch_bounds__(256) void k_cavity_step(const StepArgs a) {
         constexpr int al = decltype(A)::value;
         S::st(dst + (long long)al * a.ps + c, f[al], a.scale);
     });
+}
+
+struct LeavingTable {
+    int pop[NFACE][5];
+};
+constexpr LeavingTable make_leaving() {
+    LeavingTable t{};
+    for (int f = 0; f < NFACE; ++f)
+        for (int q = 0; q < 5; ++q) t.pop[f][q] = leaving_pop(f, q);
+    return t;
+}
+constexpr LeavingTable LEAVING = make_leaving();
+
+// Pack every outgoing region of state `src` into `buf` (packed layout).
+// grid.y = region; grid.x covers the largest region.
+template <class T>
+__global__ __launch_bounds__(256) void k_halo_pack(const T* __restrict__ src, T* __restrict__ buf,
+                                                   const StepArgs a) {
+    const int r = blockIdx.y;
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    int i, j, k, al;
+    if (r < NFACE) {
+        const int axis = face_axis(r), hi = r & 1;
+        const long long plane = axis == 0 ? (long long)a.ny * a.nz
+                                          : (axis == 1 ? (long long)a.nx * a.nz : (long long)a.nx * a.ny);
+        if (e >= 5 * plane) return;
+        const int slot = int(e / plane);
+        const long long p = e % plane;
+        if (axis == 0) { i = hi ? a.nx - 1 : 0; j = int(p % a.ny); k = int(p / a.ny); }
+        else if (axis == 1) { j = hi ? a.ny - 1 : 0; i = int(p % a.nx); k = int(p / a.nx); }
+        else { k = hi ? a.nz - 1 : 0; i = int(p % a.nx); j = int(p / a.nx); }
+        al = LEAVING.pop[r][slot];
+    } else {
+        al = r - NFACE + 7;
+        const int fa = edge_free_axis(al);
+        const int n = fa == 0 ? a.nx : (fa == 1 ? a.ny : a.nz);
+        if (e >= n) return;
+        const int t = int(e);
+        i = EX[al] > 0 ? a.nx - 1 : (EX[al] < 0 ? 0 : t);
+        j = EY[al] > 0 ? a.ny - 1 : (EY[al] < 0 ? 0 : t);
+        k = EZ[al] > 0 ? a.nz - 1 : (EZ[al] < 0 ? 0 : t);
+    }
+    const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
+    buf[a.hoff[r] + e] = src[(long long)al * a.ps + c];
+}
+
+// Take incoming regions from `recv` into the context's halo storage.  With
+// loopback, face r takes what this subdomain packed for the opposite face
+// (r ^ 1) and each edge its own packed line: the exchange wrapped around on
+// one GPU.
+template <class T>
+__global__ __launch_bounds__(256) void k_halo_unpack(const T* __restrict__ recv, T* __restrict__ halo,
+                                                     const StepArgs a, int loopback) {
+    const int r = blockIdx.y;
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= a.hcount[r]) return;
+    const int from = (loopback && r < NFACE) ? (r ^ 1) : r;
+    halo[a.hoff[r] + e] = recv[a.hoff[from] + e];
 }
 
 // Equilibrium start (initial.f90:63-73) followed by the first collision with
@@ -198,8 +290,6 @@ __global__ __launch_bounds__(256) void k_cavity_init(const StepArgs a, const dou
 }
 
 // ---- FP64 residual (check.f90:9-34) -----------------------------------------------
-constexpr int RED_THREADS = 256;
-constexpr int RED_BLOCKS = 1024;
 
 __device__ __forceinline__ double wave_sum(double x) {
     for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
@@ -277,39 +367,38 @@ __global__ void k_fill(double* __restrict__ p, long long n, double val) {
         p[c] = val;
 }
 
+// T of the thermal initial state: t_hot on the first layer along `axis`,
+// t_cold on the last, 0 elsewhere (axis < 0: 0 everywhere).
+__global__ void k_fill_layers(double* __restrict__ T, int nx, int ny, int nz, int axis, double t_hot,
+                              double t_cold) {
+    const long long n = (long long)nx * ny * nz;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
+         c += (long long)gridDim.x * blockDim.x) {
+        const long long i = c % nx, j = (c / nx) % ny, k = c / ((long long)nx * ny);
+        const long long q = axis == 0 ? i : (axis == 1 ? j : k);
+        const long long nq = axis == 0 ? nx : (axis == 1 ? ny : nz);
+        double t = 0.0;
+        if (axis >= 0 && q == 0) t = t_hot;
+        if (axis >= 0 && q == nq - 1) t = t_cold;
+        T[c] = t;
+    }
+}
+
 using StepFn = void (*)(const StepArgs);
 using InitFn = void (*)(const StepArgs, const double*, const double*, const double*, const double*);
 
 template <class S>
 struct Table {
-    static constexpr StepFn step[2][2] = {
-        {k_cavity_step<S, COLL_MRT, false>, k_cavity_step<S, COLL_MRT, true>},
-        {k_cavity_step<S, COLL_BGK, false>, k_cavity_step<S, COLL_BGK, true>}};
+    static constexpr StepFn step[2][2][2] = {
+        {{k_cavity_step<S, COLL_MRT, false, false>, k_cavity_step<S, COLL_MRT, false, true>},
+         {k_cavity_step<S, COLL_MRT, true, false>, k_cavity_step<S, COLL_MRT, true, true>}},
+        {{k_cavity_step<S, COLL_BGK, false, false>, k_cavity_step<S, COLL_BGK, false, true>},
+         {k_cavity_step<S, COLL_BGK, true, false>, k_cavity_step<S, COLL_BGK, true, true>}}};
     static constexpr InitFn init[2] = {k_cavity_init<S, COLL_MRT>, k_cavity_init<S, COLL_BGK>};
 };
 
 }  // namespace
 
-struct lbm_ctx {
-    lbm_params p{};
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int px = 0;
-    long long pxy = 0, ps = 0;
-    size_t elem = 8;
-    void* buf[2] = {nullptr, nullptr};
-    int cur = 0;                 // buffer holding the current f_post
-    double* lid_rho = nullptr;
-    double* macro = nullptr;     // rho, u, v, w (4 * n)
-    double* prev = nullptr;      // up, vp (2 * n)
-    double* red = nullptr;       // 2 * RED_BLOCKS partials + 2 totals
-    long long n = 0;
-    long long itc = 0;
-    bool initialised = false;
-    dim3 block{64, 4, 1};
-    float scale = 1.0f, inv_scale = 1.0f;
-    long long device_bytes = 0;
-};
 
 namespace {
 
@@ -332,6 +421,12 @@ StepArgs make_args(const lbm_ctx* c) {
     a.u = c->macro + c->n;
     a.v = c->macro + 2 * c->n;
     a.w = c->macro + 3 * c->n;
+    a.halo_faces = c->p.halo_faces;
+    a.halo = c->hin;
+    for (int r = 0; r < NREGION; ++r) {
+        a.hoff[r] = c->hoff[r];
+        a.hcount[r] = c->hcount[r];
+    }
     return a;
 }
 
@@ -342,11 +437,56 @@ dim3 grid_of(const lbm_ctx* c) {
 
 StepFn step_fn(const lbm_ctx* c, bool macro) {
     const int coll = c->p.collision == LBM_BGK ? 1 : 0;
+    const int halo = c->p.halo_faces != 0 ? 1 : 0;
     switch (c->p.storage) {
-    case LBM_STORE_F32: return Table<StoreF32>::step[coll][macro];
-    case LBM_STORE_F16: return Table<StoreF16>::step[coll][macro];
-    default: return Table<StoreF64>::step[coll][macro];
+    case LBM_STORE_F32: return Table<StoreF32>::step[coll][macro][halo];
+    case LBM_STORE_F16: return Table<StoreF16>::step[coll][macro][halo];
+    default: return Table<StoreF64>::step[coll][macro][halo];
     }
+}
+
+long long halo_max_region(const lbm_ctx* c) {
+    long long m = 0;
+    for (int r = 0; r < NREGION; ++r) m = c->hcount[r] > m ? c->hcount[r] : m;
+    return m;
+}
+
+// Launch the pack (dst = send buffer) or unpack (src = receive buffer) kernel
+// for the context's storage type.
+int launch_pack(lbm_ctx* c, const void* state, void* send) {
+    const StepArgs a = make_args(c);
+    const dim3 grid((unsigned)((halo_max_region(c) + 255) / 256), NREGION);
+    switch (c->p.storage) {
+    case LBM_STORE_F32:
+        hipLaunchKernelGGL(k_halo_pack<float>, grid, dim3(256), 0, c->stream, (const float*)state, (float*)send, a);
+        break;
+    case LBM_STORE_F16:
+        hipLaunchKernelGGL(k_halo_pack<_Float16>, grid, dim3(256), 0, c->stream, (const _Float16*)state,
+                           (_Float16*)send, a);
+        break;
+    default:
+        hipLaunchKernelGGL(k_halo_pack<double>, grid, dim3(256), 0, c->stream, (const double*)state, (double*)send, a);
+    }
+    return lbm_check_launch();
+}
+
+int launch_unpack(lbm_ctx* c, const void* recv, int loopback) {
+    const StepArgs a = make_args(c);
+    const dim3 grid((unsigned)((halo_max_region(c) + 255) / 256), NREGION);
+    switch (c->p.storage) {
+    case LBM_STORE_F32:
+        hipLaunchKernelGGL(k_halo_unpack<float>, grid, dim3(256), 0, c->stream, (const float*)recv, (float*)c->hin,
+                           a, loopback);
+        break;
+    case LBM_STORE_F16:
+        hipLaunchKernelGGL(k_halo_unpack<_Float16>, grid, dim3(256), 0, c->stream, (const _Float16*)recv,
+                           (_Float16*)c->hin, a, loopback);
+        break;
+    default:
+        hipLaunchKernelGGL(k_halo_unpack<double>, grid, dim3(256), 0, c->stream, (const double*)recv, (double*)c->hin,
+                           a, loopback);
+    }
+    return lbm_check_launch();
 }
 
 InitFn init_fn(const lbm_ctx* c) {
@@ -358,12 +498,6 @@ InitFn init_fn(const lbm_ctx* c) {
     }
 }
 
-int check_launch() {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(LBM_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
-    return LBM_OK;
-}
-
 void release(lbm_ctx* c) {
     for (void* b : c->buf)
         if (b) (void)hipFree(b);
@@ -371,9 +505,30 @@ void release(lbm_ctx* c) {
     if (c->macro) (void)hipFree(c->macro);
     if (c->prev) (void)hipFree(c->prev);
     if (c->red) (void)hipFree(c->red);
+    if (c->hin) (void)hipFree(c->hin);
+    if (c->hsend) (void)hipFree(c->hsend);
+    thermal_release(c);
 }
 
 }  // namespace
+
+namespace {
+thread_local char g_err[512] = "";
+}  // namespace
+
+int lbm_fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int lbm_check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return lbm_fail(LBM_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    return LBM_OK;
+}
 
 extern "C" {
 
@@ -394,26 +549,45 @@ const char* lbm_status_string(int status) {
 }
 
 int lbm_create(const lbm_params* p, lbm_ctx** out) {
-    if (!p || !out) return fail(LBM_ERR_ARG, "lbm_create: null argument");
+    if (!p || !out) return lbm_fail(LBM_ERR_ARG, "lbm_create: null argument");
     *out = nullptr;
     if (p->nx < 2 || p->ny < 2 || p->nz < 2)
-        return fail(LBM_ERR_ARG, "lbm_create: grid %d x %d x %d too small", p->nx, p->ny, p->nz);
+        return lbm_fail(LBM_ERR_ARG, "lbm_create: grid %d x %d x %d too small", p->nx, p->ny, p->nz);
     if (!(p->tau > 0.5) || !std::isfinite(p->tau))
-        return fail(LBM_ERR_TAU, "lbm_create: tau = %g must be > 0.5", p->tau);
+        return lbm_fail(LBM_ERR_TAU, "lbm_create: tau = %g must be > 0.5", p->tau);
     if (p->collision != LBM_MRT && p->collision != LBM_BGK)
-        return fail(LBM_ERR_ARG, "lbm_create: collision %d unknown", p->collision);
+        return lbm_fail(LBM_ERR_ARG, "lbm_create: collision %d unknown", p->collision);
     if (p->storage < LBM_STORE_F64 || p->storage > LBM_STORE_F16)
-        return fail(LBM_ERR_ARG, "lbm_create: storage %d unknown", p->storage);
-    if (p->thermal != 0) return fail(LBM_ERR_UNSUPPORTED, "lbm_create: thermal coupling not built");
+        return lbm_fail(LBM_ERR_ARG, "lbm_create: storage %d unknown", p->storage);
+    if (p->thermal != 0 && p->thermal != 1) return lbm_fail(LBM_ERR_ARG, "lbm_create: thermal = %d", p->thermal);
+    if (p->thermal) {
+        if (p->storage != LBM_STORE_F64)
+            return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_create: the thermal path stores FP64 populations only");
+        if (p->halo_faces || p->halo_loopback)
+            return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_create: the thermal path has no halo exchange");
+        if (p->u_lid != 0.0)
+            return lbm_fail(LBM_ERR_ARG, "lbm_create: the buoyancy cavity has no moving lid (u_lid = %g)", p->u_lid);
+        if (p->thermal_axis < -1 || p->thermal_axis > 2)
+            return lbm_fail(LBM_ERR_ARG, "lbm_create: thermal_axis %d", p->thermal_axis);
+        if (!(p->para_a + 6.0 > 0.0))
+            return lbm_fail(LBM_ERR_TAU, "lbm_create: paraA = %g gives a non-positive diffusivity", p->para_a);
+    }
     if (p->f16_scale_log2 < 0 || p->f16_scale_log2 > 20)
-        return fail(LBM_ERR_ARG, "lbm_create: f16_scale_log2 %d out of [0, 20]", p->f16_scale_log2);
-    const int bx = p->block_x ? p->block_x : 64, by = p->block_y ? p->block_y : 4;
+        return lbm_fail(LBM_ERR_ARG, "lbm_create: f16_scale_log2 %d out of [0, 20]", p->f16_scale_log2);
+    // default launch shape: one block per 256-cell x-row run (measured best at
+    // 256^3: whole rows per workgroup share their partial edge cache lines)
+    const int bx_def = p->nx >= 256 ? 256 : (p->nx + 63) / 64 * 64;
+    const int bx = p->block_x ? p->block_x : bx_def;
+    const int by = p->block_y ? p->block_y : 256 / bx;
     if (bx % 64 != 0 || bx * by > 256 || by < 1)
-        return fail(LBM_ERR_ARG, "lbm_create: block %d x %d (x must be a multiple of 64, <= 256 threads)",
+        return lbm_fail(LBM_ERR_ARG, "lbm_create: block %d x %d (x must be a multiple of 64, <= 256 threads)",
                     bx, by);
 
+    if (p->halo_faces < 0 || p->halo_faces > 63)
+        return lbm_fail(LBM_ERR_ARG, "lbm_create: halo_faces 0x%x is not a 6-bit face mask", p->halo_faces);
+
     lbm_ctx* c = new (std::nothrow) lbm_ctx();
-    if (!c) return fail(LBM_ERR_NOMEM, "lbm_create: host allocation failed");
+    if (!c) return lbm_fail(LBM_ERR_NOMEM, "lbm_create: host allocation failed");
     c->p = *p;
     if (c->p.rho0 == 0.0) c->p.rho0 = 1.0;
     c->block = dim3(bx, by, 1);
@@ -423,7 +597,7 @@ int lbm_create(const lbm_params* p, lbm_ctx** out) {
     const long long cells = c->pxy * p->nz;
     if (cells >= (1LL << 31)) {
         delete c;
-        return fail(LBM_ERR_ARG, "lbm_create: %lld padded cells exceed the 32-bit cell index", cells);
+        return lbm_fail(LBM_ERR_ARG, "lbm_create: %lld padded cells exceed the 32-bit cell index", cells);
     }
     // population stride: whole 4 KiB pages plus a 1 KiB stagger per stream
     const long long page = 4096 / (long long)c->elem;
@@ -433,6 +607,26 @@ int lbm_create(const lbm_params* p, lbm_ctx** out) {
     c->scale = p->storage == LBM_STORE_F16 ? std::ldexp(1.0f, k) : 1.0f;
     c->inv_scale = 1.0f / c->scale;
 
+    // packed halo layout: 6 faces (5 populations x face plane), 12 edge lines
+    {
+        long long off = 0;
+        for (int r = 0; r < NREGION; ++r) {
+            long long n;
+            if (r < NFACE) {
+                const int axis = face_axis(r);
+                n = 5LL * (axis == 0 ? (long long)p->ny * p->nz
+                                     : (axis == 1 ? (long long)p->nx * p->nz : (long long)p->nx * p->ny));
+            } else {
+                const int fa = edge_free_axis(r - NFACE + 7);
+                n = fa == 0 ? p->nx : (fa == 1 ? p->ny : p->nz);
+            }
+            c->hoff[r] = off;
+            c->hcount[r] = n;
+            off += n;
+        }
+        c->htotal = off;
+    }
+
     (void)hipGetDevice(&c->device);
     const size_t lat = (size_t)c->ps * Q * c->elem;
     hipError_t e = hipSuccess;
@@ -441,14 +635,32 @@ int lbm_create(const lbm_params* p, lbm_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(&c->lid_rho, sizeof(double) * p->nx * p->ny);
     if (e == hipSuccess) e = hipMalloc(&c->macro, sizeof(double) * 4 * c->n);
     if (e == hipSuccess) e = hipMalloc(&c->prev, sizeof(double) * 2 * c->n);
-    if (e == hipSuccess) e = hipMalloc(&c->red, sizeof(double) * (2 * RED_BLOCKS + 2));
+    if (e == hipSuccess) e = hipMalloc(&c->red, sizeof(double) * RED_MAX_SUMS * (RED_BLOCKS + 1));
+    if (e == hipSuccess) e = hipMalloc(&c->hin, c->elem * c->htotal);
+    if (e == hipSuccess) e = hipMalloc(&c->hsend, c->elem * c->htotal);
     if (e != hipSuccess) {
         release(c);
         delete c;
         (void)hipGetLastError();
-        return fail(LBM_ERR_NOMEM, "lbm_create: hipMalloc failed: %s", hipGetErrorString(e));
+        return lbm_fail(LBM_ERR_NOMEM, "lbm_create: hipMalloc failed: %s", hipGetErrorString(e));
     }
-    c->device_bytes = (long long)(2 * lat + sizeof(double) * (p->nx * p->ny + 6 * c->n + 2 * RED_BLOCKS + 2));
+    c->device_bytes = (long long)(2 * lat + sizeof(double) * (p->nx * p->ny + 6 * c->n) +
+                                  sizeof(double) * RED_MAX_SUMS * (RED_BLOCKS + 1) + 2 * c->elem * c->htotal);
+    if (p->thermal) {
+        int rc = thermal_alloc(c);
+        if (rc) {
+            (void)hipGetLastError();
+            release(c);
+            delete c;
+            return rc == LBM_ERR_HIP ? LBM_ERR_NOMEM : rc;
+        }
+    }
+    // incoming halos start as zeros until the first exchange
+    if (hipMemset(c->hin, 0, c->elem * c->htotal) != hipSuccess) {
+        release(c);
+        delete c;
+        return lbm_fail(LBM_ERR_HIP, "lbm_create: clearing the halo failed");
+    }
     *out = c;
     return LBM_OK;
 }
@@ -462,22 +674,29 @@ int lbm_destroy(lbm_ctx* c) {
 }
 
 int lbm_set_stream(lbm_ctx* c, void* stream) {
-    if (!c) return fail(LBM_ERR_ARG, "lbm_set_stream: null context");
+    if (!c) return lbm_fail(LBM_ERR_ARG, "lbm_set_stream: null context");
     c->stream = static_cast<hipStream_t>(stream);
     return LBM_OK;
 }
 
 static int load_device_fields(lbm_ctx* c, const double* d_rho, const double* d_u, const double* d_v,
                               const double* d_w) {
-    StepArgs a = make_args(c);
-    a.dst = c->buf[0];
-    hipLaunchKernelGGL(init_fn(c), grid_of(c), c->block, 0, c->stream, a, d_rho, d_u, d_v, d_w);
-    int rc = check_launch();
+    int rc;
+    if (c->p.thermal) {
+        rc = thermal_load(c, d_rho, d_u, d_v, d_w, c->temp);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, c->stream, c->prev_t, 2 * c->n, 0.0);
+    } else {
+        StepArgs a = make_args(c);
+        a.dst = c->buf[0];
+        hipLaunchKernelGGL(init_fn(c), grid_of(c), c->block, 0, c->stream, a, d_rho, d_u, d_v, d_w);
+    }
+    rc = lbm_check_launch();
     if (rc) return rc;
     // up = vp = 0 (initial.f90:51-53); a fill kernel rather than a memset so
     // the sequence stays valid under stream capture
     hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, c->stream, c->prev, 2 * c->n, 0.0);
-    rc = check_launch();
+    rc = lbm_check_launch();
     if (rc) return rc;
     c->cur = 0;
     c->itc = 0;
@@ -486,7 +705,7 @@ static int load_device_fields(lbm_ctx* c, const double* d_rho, const double* d_u
 }
 
 int lbm_initial(lbm_ctx* c) {
-    if (!c) return fail(LBM_ERR_ARG, "lbm_initial: null context");
+    if (!c) return lbm_fail(LBM_ERR_ARG, "lbm_initial: null context");
     // initial fields into the macro arrays, then equilibrium + first collision
     // from them.  The init kernel reads and rewrites the same macro values
     // cell by cell, so the arrays can serve as both input and output.
@@ -495,17 +714,31 @@ int lbm_initial(lbm_ctx* c) {
     hipLaunchKernelGGL(k_fill, dim3(nb), dim3(tb), 0, c->stream, c->macro, n, c->p.rho0);
     hipLaunchKernelGGL(k_fill, dim3(nb), dim3(tb), 0, c->stream, c->macro + n, n, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(nb), dim3(tb), 0, c->stream, c->macro + 2 * n, 2 * n, 0.0);
-    // u = U0 on the top fluid layer k = nz (initial.f90:54-61)
-    hipLaunchKernelGGL(k_fill, dim3(64), dim3(tb), 0, c->stream, c->macro + n + (n - layer), layer,
-                       c->p.u_lid);
-    int rc = check_launch();
+    // u = U0 on the top fluid layer k = nz of the subdomain that holds the lid
+    // (initial.f90:54-61 tests coords(2) == dims(2) - 1; here: z+ is a wall)
+    if (!((c->p.halo_faces >> LBM_FACE_ZP) & 1))
+        hipLaunchKernelGGL(k_fill, dim3(64), dim3(tb), 0, c->stream, c->macro + n + (n - layer), layer,
+                           c->p.u_lid);
+    if (c->p.thermal) {
+        // T = 0 except the layers next to the held walls (initial.F90:126-162)
+        hipLaunchKernelGGL(k_fill_layers, dim3(2048), dim3(256), 0, c->stream, c->temp, c->p.nx, c->p.ny, c->p.nz,
+                           c->p.thermal_axis, c->p.t_hot, c->p.t_cold);
+    }
+    int rc = lbm_check_launch();
     if (rc) return rc;
     return load_device_fields(c, c->macro, c->macro + n, c->macro + 2 * n, c->macro + 3 * n);
 }
 
-int lbm_load_fields(lbm_ctx* c, const double* rho, const double* u, const double* v, const double* w) {
-    if (!c || !rho || !u || !v || !w) return fail(LBM_ERR_ARG, "lbm_load_fields: null argument");
+int lbm_load_fields(lbm_ctx* c, const double* rho, const double* u, const double* v, const double* w,
+                    const double* T) {
+    if (!c || !rho || !u || !v || !w) return lbm_fail(LBM_ERR_ARG, "lbm_load_fields: null argument");
     const size_t bytes = sizeof(double) * c->n;
+    if (c->p.thermal) {
+        if (T)
+            HIP_TRY(hipMemcpyAsync(c->temp, T, bytes, hipMemcpyHostToDevice, c->stream));
+        else
+            hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, c->stream, c->temp, c->n, 0.0);
+    }
     HIP_TRY(hipMemcpyAsync(c->macro, rho, bytes, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->macro + c->n, u, bytes, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->macro + 2 * c->n, v, bytes, hipMemcpyHostToDevice, c->stream));
@@ -517,42 +750,128 @@ int lbm_load_fields(lbm_ctx* c, const double* rho, const double* u, const double
 }
 
 int lbm_step(lbm_ctx* c, int nsteps) {
-    if (!c) return fail(LBM_ERR_ARG, "lbm_step: null context");
-    if (nsteps < 0) return fail(LBM_ERR_ARG, "lbm_step: nsteps = %d", nsteps);
-    if (!c->initialised) return fail(LBM_ERR_STATE, "lbm_step before lbm_initial / lbm_load_fields");
+    if (!c) return lbm_fail(LBM_ERR_ARG, "lbm_step: null context");
+    if (nsteps < 0) return lbm_fail(LBM_ERR_ARG, "lbm_step: nsteps = %d", nsteps);
+    if (!c->initialised) return lbm_fail(LBM_ERR_STATE, "lbm_step before lbm_initial / lbm_load_fields");
+    if (c->p.thermal) {
+        int rc = thermal_step(c, nsteps);
+        c->itc += nsteps;
+        return rc;
+    }
+    if (c->p.halo_faces != 0 && !c->p.halo_loopback && nsteps > 1)
+        return lbm_fail(LBM_ERR_STATE, "lbm_step: with halo faces and no loopback the halo must be exchanged "
+                                   "between steps; call lbm_step(ctx, 1)");
     StepArgs a = make_args(c);
     const dim3 grid = grid_of(c);
     for (int s = 0; s < nsteps; ++s) {
+        if (c->p.halo_loopback) {
+            int rc = launch_pack(c, c->buf[c->cur], c->hsend);
+            if (!rc) rc = launch_unpack(c, c->hsend, 1);
+            if (rc) return rc;
+        }
         a.src = c->buf[c->cur];
         a.dst = c->buf[1 - c->cur];
         hipLaunchKernelGGL(step_fn(c, s == nsteps - 1), grid, c->block, 0, c->stream, a);
         c->cur = 1 - c->cur;
     }
     c->itc += nsteps;
-    return check_launch();
+    return lbm_check_launch();
 }
 
-int lbm_check(lbm_ctx* c, double* error_u) {
-    if (!c || !error_u) return fail(LBM_ERR_ARG, "lbm_check: null argument");
-    if (!c->initialised) return fail(LBM_ERR_STATE, "lbm_check before lbm_initial");
+int lbm_check_sums(lbm_ctx* c, double* e1, double* e2) {
+    if (!c || !e1 || !e2) return lbm_fail(LBM_ERR_ARG, "lbm_check_sums: null argument");
+    if (!c->initialised) return lbm_fail(LBM_ERR_STATE, "lbm_check before lbm_initial");
+    if (c->p.thermal) {
+        double s4[4];
+        int rc = thermal_check(c, s4);
+        if (rc) return rc;
+        *e1 = s4[0];
+        *e2 = s4[1];
+        return LBM_OK;
+    }
     const long long n = c->n;
     hipLaunchKernelGGL(k_check_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream,
                        c->macro + n, c->macro + 2 * n, c->macro + 3 * n, c->prev, c->prev + n, n, c->red);
     hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(RED_THREADS), 0, c->stream, c->red, RED_BLOCKS,
                        c->red + 2 * RED_BLOCKS);
-    int rc = check_launch();
+    int rc = lbm_check_launch();
     if (rc) return rc;
     double tot[2];
     HIP_TRY(hipMemcpyAsync(tot, c->red + 2 * RED_BLOCKS, sizeof tot, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    *error_u = std::sqrt(tot[0]) / std::sqrt(tot[1]);
-    if (std::isnan(*error_u)) return fail(LBM_ERR_NAN, "lbm_check: residual is NaN at itc = %lld", c->itc);
+    *e1 = tot[0];
+    *e2 = tot[1];
     return LBM_OK;
 }
 
+int lbm_check(lbm_ctx* c, double* error_u) {
+    if (!error_u) return lbm_fail(LBM_ERR_ARG, "lbm_check: null argument");
+    double e1 = 0.0, e2 = 0.0;
+    int rc = lbm_check_sums(c, &e1, &e2);
+    if (rc) return rc;
+    *error_u = std::sqrt(e1) / std::sqrt(e2);
+    if (std::isnan(*error_u)) return lbm_fail(LBM_ERR_NAN, "lbm_check: residual is NaN at itc = %lld", c->itc);
+    return LBM_OK;
+}
+
+int lbm_check_thermal(lbm_ctx* c, double* error_u, double* error_t) {
+    if (!c || !error_u || !error_t) return lbm_fail(LBM_ERR_ARG, "lbm_check_thermal: null argument");
+    if (!c->p.thermal) return lbm_fail(LBM_ERR_STATE, "lbm_check_thermal on an isothermal context");
+    if (!c->initialised) return lbm_fail(LBM_ERR_STATE, "lbm_check_thermal before lbm_initial");
+    double s4[4];
+    int rc = thermal_check(c, s4);
+    if (rc) return rc;
+    *error_u = std::sqrt(s4[0]) / std::sqrt(s4[1]);
+    *error_t = s4[2] / s4[3];
+    if (std::isnan(*error_u) || std::isnan(*error_t))
+        return lbm_fail(LBM_ERR_NAN, "lbm_check_thermal: residual is NaN at itc = %lld", c->itc);
+    return LBM_OK;
+}
+
+int lbm_nusselt(lbm_ctx* c, double* nu_vol, double* nu_wall) {
+    if (!c || !nu_vol || !nu_wall) return lbm_fail(LBM_ERR_ARG, "lbm_nusselt: null argument");
+    if (!c->p.thermal) return lbm_fail(LBM_ERR_STATE, "lbm_nusselt on an isothermal context");
+    if (!c->initialised) return lbm_fail(LBM_ERR_STATE, "lbm_nusselt before lbm_initial");
+    return thermal_nusselt(c, nu_vol, nu_wall);
+}
+
+int lbm_download_temperature(lbm_ctx* c, double* T) {
+    if (!c || !T) return lbm_fail(LBM_ERR_ARG, "lbm_download_temperature: null argument");
+    if (!c->p.thermal) return lbm_fail(LBM_ERR_STATE, "lbm_download_temperature on an isothermal context");
+    if (!c->initialised) return lbm_fail(LBM_ERR_STATE, "lbm_download_temperature before lbm_initial");
+    HIP_TRY(hipMemcpyAsync(T, c->temp, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_halo_layout(const lbm_ctx* c, int region, long long* offset, long long* count) {
+    if (!c || !offset || !count) return lbm_fail(LBM_ERR_ARG, "lbm_halo_layout: null argument");
+    if (region < 0 || region >= NREGION) return lbm_fail(LBM_ERR_ARG, "lbm_halo_layout: region %d", region);
+    *offset = c->hoff[region];
+    *count = c->hcount[region];
+    return LBM_OK;
+}
+
+int lbm_halo_buffer_bytes(const lbm_ctx* c, size_t* bytes) {
+    if (!c || !bytes) return lbm_fail(LBM_ERR_ARG, "lbm_halo_buffer_bytes: null argument");
+    *bytes = c->elem * (size_t)c->htotal;
+    return LBM_OK;
+}
+
+int lbm_halo_pack(lbm_ctx* c, void* dev_send) {
+    if (!c || !dev_send) return lbm_fail(LBM_ERR_ARG, "lbm_halo_pack: null argument");
+    if (!c->initialised) return lbm_fail(LBM_ERR_STATE, "lbm_halo_pack before lbm_initial");
+    return launch_pack(c, c->buf[c->cur], dev_send);
+}
+
+int lbm_halo_unpack(lbm_ctx* c, const void* dev_recv) {
+    if (!c || !dev_recv) return lbm_fail(LBM_ERR_ARG, "lbm_halo_unpack: null argument");
+    return launch_unpack(c, dev_recv, 0);
+}
+
 int lbm_download_macro(lbm_ctx* c, double* rho, double* u, double* v, double* w) {
-    if (!c) return fail(LBM_ERR_ARG, "lbm_download_macro: null context");
-    if (!c->initialised) return fail(LBM_ERR_STATE, "lbm_download_macro before lbm_initial");
+    if (!c) return lbm_fail(LBM_ERR_ARG, "lbm_download_macro: null context");
+    if (!c->initialised) return lbm_fail(LBM_ERR_STATE, "lbm_download_macro before lbm_initial");
     double* dst[4] = {rho, u, v, w};
     const size_t bytes = sizeof(double) * c->n;
     for (int q = 0; q < 4; ++q)
@@ -563,13 +882,13 @@ int lbm_download_macro(lbm_ctx* c, double* rho, double* u, double* v, double* w)
 }
 
 int lbm_synchronize(lbm_ctx* c) {
-    if (!c) return fail(LBM_ERR_ARG, "lbm_synchronize: null context");
+    if (!c) return lbm_fail(LBM_ERR_ARG, "lbm_synchronize: null context");
     HIP_TRY(hipStreamSynchronize(c->stream));
     return LBM_OK;
 }
 
 int lbm_get_info(const lbm_ctx* c, lbm_info* info) {
-    if (!c || !info) return fail(LBM_ERR_ARG, "lbm_get_info: null argument");
+    if (!c || !info) return lbm_fail(LBM_ERR_ARG, "lbm_get_info: null argument");
     info->nx = c->p.nx;
     info->ny = c->p.ny;
     info->nz = c->p.nz;

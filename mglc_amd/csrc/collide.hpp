@@ -40,13 +40,16 @@ __device__ __forceinline__ R mac(R acc, R x) {
 }
 
 // ---- storage formats ----------------------------------------------------------
+// Stores are non-temporal: the next step reads the written lattice only after
+// the whole sweep, so caching the lines on the way out buys nothing (measured
+// 2-4% faster at 256^3).
 struct StoreF64 {
     using T = double;
     using Real = double;
     static constexpr bool SHIFT = false;
     static constexpr int BYTES = 8;
     __device__ __forceinline__ static Real ld(const T* p, float) { return *p; }
-    __device__ __forceinline__ static void st(T* p, Real v, float) { *p = v; }
+    __device__ __forceinline__ static void st(T* p, Real v, float) { __builtin_nontemporal_store(v, p); }
 };
 
 struct StoreF32 {
@@ -55,7 +58,7 @@ struct StoreF32 {
     static constexpr bool SHIFT = true;
     static constexpr int BYTES = 4;
     __device__ __forceinline__ static Real ld(const T* p, float) { return *p; }
-    __device__ __forceinline__ static void st(T* p, Real v, float) { *p = v; }
+    __device__ __forceinline__ static void st(T* p, Real v, float) { __builtin_nontemporal_store(v, p); }
 };
 
 // IEEE binary16 holding (f - w) * 2^k.  The scale is an exact power of two,
@@ -69,7 +72,7 @@ struct StoreF16 {
         return static_cast<float>(*p) * inv_scale;
     }
     __device__ __forceinline__ static void st(T* p, Real v, float scale) {
-        *p = static_cast<_Float16>(v * scale);
+        __builtin_nontemporal_store(static_cast<_Float16>(v * scale), p);
     }
 };
 
@@ -238,6 +241,98 @@ __device__ __forceinline__ void collide_mrt(R f[Q], const Moments<R, SHIFT>& q, 
         else if constexpr (kind == 2) m[k] = m[k] - sq * (m[k] - meq[k]);
     });
     mrt_inverse(m, f);
+}
+
+}  // namespace lbm
+
+// ---- buoyancy-driven flow (evolution_f.F90, evolution_g.F90) -------------------------
+namespace lbm {
+
+// MRT with the Guo source in moment space (evolution_f.F90:58-129):
+// m* = m - s (m - meq) + (1 - s/2) S, meq in the thermal form (meq(12) with rho).
+// Absolute (unshifted) FP64 arithmetic.
+__device__ __forceinline__ void collide_mrt_forced(double f[Q], double rho, double u, double v, double w,
+                                                   double Fx, double Fy, double Fz, double snu, double sq) {
+    double m[Q], meq[Q], S[Q];
+    mrt_forward(f, m);
+    const double us2 = u * u + v * v + w * w;
+    meq[0] = rho;
+    meq[1] = -11.0 * rho + 19.0 * rho * us2;
+    meq[2] = 3.0 * rho - 11.0 / 2.0 * rho * us2;
+    meq[3] = rho * u;
+    meq[4] = -2.0 / 3.0 * meq[3];
+    meq[5] = rho * v;
+    meq[6] = -2.0 / 3.0 * meq[5];
+    meq[7] = rho * w;
+    meq[8] = -2.0 / 3.0 * meq[7];
+    meq[9] = rho * (2.0 * u * u - v * v - w * w);
+    meq[10] = -0.5 * meq[9];
+    meq[11] = rho * (v * v - w * w);
+    meq[12] = -0.5 * meq[11];
+    meq[13] = rho * (u * v);
+    meq[14] = rho * (v * w);
+    meq[15] = rho * (w * u);
+    meq[16] = meq[17] = meq[18] = 0.0;
+    const double uF = u * Fx + v * Fy + w * Fz;
+    S[0] = 0.0;
+    S[1] = 38.0 * uF;
+    S[2] = -11.0 * uF;
+    S[3] = Fx;
+    S[4] = -2.0 / 3.0 * Fx;
+    S[5] = Fy;
+    S[6] = -2.0 / 3.0 * Fy;
+    S[7] = Fz;
+    S[8] = -2.0 / 3.0 * Fz;
+    S[9] = 4.0 * u * Fx - 2.0 * v * Fy - 2.0 * w * Fz;
+    S[10] = -2.0 * u * Fx + v * Fy + w * Fz;
+    S[11] = 2.0 * v * Fy - 2.0 * w * Fz;
+    S[12] = -v * Fy + w * Fz;
+    S[13] = u * Fy + v * Fx;
+    S[14] = v * Fz + w * Fy;
+    S[15] = u * Fz + w * Fx;
+    S[16] = S[17] = S[18] = 0.0;
+    sfor<Q>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        constexpr int kind = rate_kind<k>();
+        if constexpr (kind == 0) m[k] = m[k] + S[k];
+        else if constexpr (kind == 1) m[k] = m[k] - snu * (m[k] - meq[k]) + (1.0 - 0.5 * snu) * S[k];
+        else m[k] = m[k] - sq * (m[k] - meq[k]) + (1.0 - 0.5 * sq) * S[k];
+    });
+    mrt_inverse(m, f);
+}
+
+// D3Q7 MRT for the temperature (evolution_g.F90:1-57): n = N g,
+// neq = (T, T u, T v, T w, T paraA, 0, 0), rates (0, Qd x3, Qnu x3),
+// g_post = N^-1 n*, N^-1 = N^T diag(1/|row|^2).
+__device__ __forceinline__ void collide_d3q7(double g[Q7], double T, double u, double v, double w,
+                                             double para_a, double qd, double qnu) {
+    double n[Q7];
+    sfor<Q7>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        double acc = -0.0;
+        sfor<Q7>([&](auto A) {
+            constexpr int a = decltype(A)::value;
+            constexpr int c = basis7(k, a);
+            if constexpr (c != 0) acc = mac<c>(acc, g[a]);
+        });
+        n[k] = acc;
+    });
+    const double neq[Q7] = {T, T * u, T * v, T * w, T * para_a, 0.0, 0.0};
+    sfor<Q7>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        if constexpr (k >= 1 && k <= 3) n[k] = n[k] - qd * (n[k] - neq[k]);
+        else if constexpr (k >= 4) n[k] = n[k] - qnu * (n[k] - neq[k]);
+    });
+    sfor<Q7>([&](auto A) {
+        constexpr int a = decltype(A)::value;
+        double acc = -0.0;
+        sfor<Q7>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            constexpr int c = basis7(k, a);
+            if constexpr (c != 0) acc = acc + (double(c) / double(row_norm2_7(k))) * n[k];
+        });
+        g[a] = acc;
+    });
 }
 
 }  // namespace lbm

@@ -23,6 +23,7 @@ module commondata
     character(len=8) :: collision_model = 'MRT'   ! MRT (reference) or BGK
     character(len=8) :: storage = 'F64'           ! F64, F32 or F16 populations
     integer :: block_x = 0, block_y = 0           ! launch shape, 0 = library default
+    logical :: halo_loopback = .false.            ! run the six-face exchange every step, looped back
     character(len=256) :: output_dir = '.'
     logical :: write_tecplot = .true.
     logical :: write_centerlines = .true.
@@ -33,7 +34,7 @@ module commondata
     real(8), allocatable :: u(:, :, :), v(:, :, :), w(:, :, :), rho(:, :, :)
 
     namelist /cavity/ total_nx, total_ny, total_nz, Reynolds, rho0, U0, itc_max, eps, check_every, &
-                      collision_model, storage, block_x, block_y, output_dir, write_tecplot, &
+                      collision_model, storage, block_x, block_y, halo_loopback, output_dir, write_tecplot, &
                       write_centerlines, write_binary, write_raw
 
 contains

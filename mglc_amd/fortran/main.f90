@@ -50,10 +50,19 @@ program main
     p%thermal = 0
     p%block_x = block_x
     p%block_y = block_y
+    p%halo_faces = 0
+    p%halo_loopback = merge(1, 0, halo_loopback)
+    p%thermal_axis = -1
     p%reserved_i = 0
     p%tau = tauf
     p%u_lid = U0
     p%rho0 = rho0
+    p%para_a = 0.0d0
+    p%g_beta = 0.0d0
+    p%omega_rot = 0.0d0
+    p%t_hot = 0.0d0
+    p%t_cold = 0.0d0
+    p%t_ref = 0.0d0
     p%reserved_d = 0.0d0
 
     write(*,*) "nx=", nx, ", ny=", ny, ",  nz = ", nz

@@ -23,11 +23,16 @@ module mglc_lbm_capi
         integer(c_int) :: f16_scale_log2
         integer(c_int) :: thermal
         integer(c_int) :: block_x, block_y
-        integer(c_int) :: reserved_i(6)
+        integer(c_int) :: halo_faces
+        integer(c_int) :: halo_loopback
+        integer(c_int) :: thermal_axis
+        integer(c_int) :: reserved_i(3)
         real(c_double) :: tau
         real(c_double) :: u_lid
         real(c_double) :: rho0
-        real(c_double) :: reserved_d(8)
+        real(c_double) :: para_a, g_beta, omega_rot
+        real(c_double) :: t_hot, t_cold, t_ref
+        real(c_double) :: reserved_d(2)
     end type lbm_params
 
     interface
@@ -47,11 +52,29 @@ module mglc_lbm_capi
             type(c_ptr), value :: ctx
         end function lbm_initial
 
-        integer(c_int) function lbm_load_fields(ctx, rho, u, v, w) bind(C, name='lbm_load_fields')
+        integer(c_int) function lbm_load_fields(ctx, rho, u, v, w, T) bind(C, name='lbm_load_fields')
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx
-            real(c_double), intent(in) :: rho(*), u(*), v(*), w(*)
+            real(c_double), intent(in) :: rho(*), u(*), v(*), w(*), T(*)
         end function lbm_load_fields
+
+        integer(c_int) function lbm_check_thermal(ctx, error_u, error_t) bind(C, name='lbm_check_thermal')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: error_u, error_t
+        end function lbm_check_thermal
+
+        integer(c_int) function lbm_nusselt(ctx, nu_vol, nu_wall) bind(C, name='lbm_nusselt')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: nu_vol, nu_wall
+        end function lbm_nusselt
+
+        integer(c_int) function lbm_download_temperature(ctx, T) bind(C, name='lbm_download_temperature')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: T(*)
+        end function lbm_download_temperature
 
         integer(c_int) function lbm_step(ctx, nsteps) bind(C, name='lbm_step')
             import :: c_int, c_ptr

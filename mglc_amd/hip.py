@@ -26,6 +26,10 @@ def runtime() -> ctypes.CDLL:
         h.hipStreamDestroy.argtypes = [vp]
         h.hipStreamSynchronize.argtypes = [vp]
         h.hipGetErrorString.restype = ctypes.c_char_p
+        h.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+        h.hipFree.argtypes = [vp]
+        h.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+        h.hipMemset.argtypes = [vp, ctypes.c_int, ctypes.c_size_t]
         _hip = h
     return _hip
 
@@ -69,3 +73,42 @@ class Event:
         if self.handle:
             runtime().hipEventDestroy(self.handle)
             self.handle = ctypes.c_void_p()
+
+
+def device_copy_calibration(nbytes: int, reps: int) -> None:
+    """reps device-to-device copies of nbytes (HIP's blit kernel): a plain copy
+    of known size, against which the HBM byte counters are scaled."""
+    h = runtime()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    _ok(h.hipMalloc(ctypes.byref(a), nbytes), "hipMalloc")
+    _ok(h.hipMalloc(ctypes.byref(b), nbytes), "hipMalloc")
+    _ok(h.hipMemset(a, 1, nbytes), "hipMemset")
+    for _ in range(reps):
+        _ok(h.hipMemcpy(b, a, nbytes, 3), "hipMemcpy")  # 3 = hipMemcpyDeviceToDevice
+    h.hipFree(a)
+    h.hipFree(b)
+
+
+class DeviceBuffer:
+    """A plain device allocation (for halo send/receive buffers in tests and
+    drivers that exchange between subdomains)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        _ok(runtime().hipMalloc(ctypes.byref(self.ptr), max(1, self.nbytes)), "hipMalloc")
+
+    @property
+    def address(self) -> int:
+        return int(self.ptr.value)
+
+    def copy_from(self, src: "DeviceBuffer", dst_offset: int, src_offset: int, nbytes: int):
+        if dst_offset + nbytes > self.nbytes or src_offset + nbytes > src.nbytes or min(dst_offset, src_offset) < 0:
+            raise ValueError("device copy out of bounds")
+        _ok(runtime().hipMemcpy(ctypes.c_void_p(self.address + dst_offset), ctypes.c_void_p(src.address + src_offset),
+                                nbytes, 3), "hipMemcpy")
+
+    def close(self):
+        if self.ptr:
+            runtime().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()

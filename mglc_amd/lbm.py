@@ -30,8 +30,12 @@ LBM_OK, LBM_ERR_NAN = 0, 3
 EXPORTED_SYMBOLS = (
     "lbm_create", "lbm_destroy", "lbm_set_stream", "lbm_initial", "lbm_load_fields", "lbm_step",
     "lbm_check", "lbm_download_macro", "lbm_synchronize", "lbm_get_info", "lbm_status_string",
-    "lbm_last_error",
+    "lbm_last_error", "lbm_check_sums", "lbm_halo_layout", "lbm_halo_buffer_bytes", "lbm_halo_pack",
+    "lbm_halo_unpack", "lbm_check_thermal", "lbm_nusselt", "lbm_download_temperature",
 )
+
+FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = range(6)
+NUM_REGIONS = 18
 
 
 class LbmParams(ctypes.Structure):
@@ -40,9 +44,12 @@ class LbmParams(ctypes.Structure):
         ("collision", ctypes.c_int), ("storage", ctypes.c_int), ("meq12_rho", ctypes.c_int),
         ("f16_scale_log2", ctypes.c_int), ("thermal", ctypes.c_int),
         ("block_x", ctypes.c_int), ("block_y", ctypes.c_int),
-        ("reserved_i", ctypes.c_int * 6),
+        ("halo_faces", ctypes.c_int), ("halo_loopback", ctypes.c_int), ("thermal_axis", ctypes.c_int),
+        ("reserved_i", ctypes.c_int * 3),
         ("tau", ctypes.c_double), ("u_lid", ctypes.c_double), ("rho0", ctypes.c_double),
-        ("reserved_d", ctypes.c_double * 8),
+        ("para_a", ctypes.c_double), ("g_beta", ctypes.c_double), ("omega_rot", ctypes.c_double),
+        ("t_hot", ctypes.c_double), ("t_cold", ctypes.c_double), ("t_ref", ctypes.c_double),
+        ("reserved_d", ctypes.c_double * 2),
     ]
 
 
@@ -77,12 +84,20 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lbm_destroy.argtypes = [vp]
     lib.lbm_set_stream.argtypes = [vp, vp]
     lib.lbm_initial.argtypes = [vp]
-    lib.lbm_load_fields.argtypes = [vp, dp, dp, dp, dp]
+    lib.lbm_load_fields.argtypes = [vp, dp, dp, dp, dp, dp]
+    lib.lbm_check_thermal.argtypes = [vp, dp, dp]
+    lib.lbm_nusselt.argtypes = [vp, dp, dp]
+    lib.lbm_download_temperature.argtypes = [vp, dp]
     lib.lbm_step.argtypes = [vp, ip]
     lib.lbm_check.argtypes = [vp, dp]
     lib.lbm_download_macro.argtypes = [vp, dp, dp, dp, dp]
     lib.lbm_synchronize.argtypes = [vp]
     lib.lbm_get_info.argtypes = [vp, ctypes.POINTER(LbmInfo)]
+    lib.lbm_check_sums.argtypes = [vp, dp, dp]
+    lib.lbm_halo_layout.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
+    lib.lbm_halo_buffer_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+    lib.lbm_halo_pack.argtypes = [vp, vp]
+    lib.lbm_halo_unpack.argtypes = [vp, vp]
     lib.lbm_status_string.argtypes = [ip]
     lib.lbm_status_string.restype = ctypes.c_char_p
     lib.lbm_last_error.restype = ctypes.c_char_p
@@ -117,6 +132,8 @@ class CavityConfig:
     f16_scale_log2: int = 0
     block_x: int = 0
     block_y: int = 0
+    halo_faces: int = 0        # bit f: face f reads a halo (FACE_XM..FACE_ZP) instead of a wall
+    halo_loopback: int = 0     # 1: exchange looped back onto this subdomain before every step
 
     def tau(self) -> float:
         if self.tauf is not None:
@@ -131,7 +148,79 @@ class CavityConfig:
         p.meq12_rho = self.meq12_rho
         p.f16_scale_log2 = self.f16_scale_log2
         p.block_x, p.block_y = self.block_x, self.block_y
+        p.halo_faces, p.halo_loopback = self.halo_faces, self.halo_loopback
         p.tau, p.u_lid, p.rho0 = self.tau(), self.U0, self.rho0
+        return p
+
+
+def splitmix_uniform(shape, seed: int, amp: float = 1e-4):
+    """Uniform numbers in [-amp, amp) from SplitMix64 of the linear cell index
+    (x fastest) plus `seed`: the seeded noise of the buoyancy case's initial
+    temperature, reproducible on any host (mglc_amd/fortran/buoyancy.f90 uses
+    the same generator)."""
+    n = int(np.prod(shape))
+    with np.errstate(over="ignore"):
+        z = (np.arange(n, dtype=np.uint64) + np.uint64(seed)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u01 = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return (amp * (2.0 * u01 - 1.0)).reshape(shape)
+
+
+def buoyancy_initial_temperature(nx, ny, nz, t_hot=0.5, t_cold=-0.5, seed=2024, amp=1e-4):
+    """Initial T of the buoyancy case: the reference's held layers next to the
+    y walls (initial.F90:126-144) and T = 0 plus seeded uniform noise in
+    [-amp, amp) on every other cell."""
+    T = np.zeros((nz, ny, nx))
+    T[:, 1:-1, :] = splitmix_uniform((nz, ny - 2, nx), seed, amp)
+    T[:, 0, :] = t_hot
+    T[:, -1, :] = t_cold
+    return T
+
+
+@dataclass
+class ThermalConfig:
+    """Buoyancy-driven cavity, parameters named as in the reference's module
+    (buoyancy .../3d/mpi_blocked/module.F90:11-33); derived constants follow
+    its parameter statements.  thermal_axis: 1 = y walls held (LeftRightWallsConstT)."""
+    nx: int
+    ny: int
+    nz: int
+    Rayleigh: float = 1e5
+    Prandtl: float = 0.71
+    Mach: float = 0.1
+    Thot: float = 0.5
+    Tcold: float = -0.5
+    Tref: float = 0.0
+    Ekman: float | None = None     # None: no Coriolis force
+    thermal_axis: int = 1
+    N: int | None = None           # length used in the parameter statements (default nz)
+    block_x: int = 0
+    block_y: int = 0
+
+    def derived(self):
+        import math
+        N = float(self.N if self.N is not None else self.nz)
+        tauf = 0.5 + self.Mach * N * math.sqrt(3.0 * self.Prandtl / self.Rayleigh)
+        viscosity = (tauf - 0.5) / 3.0
+        diffusivity = viscosity / self.Prandtl
+        omega = 0.0 if self.Ekman is None else viscosity / 2.0 / self.Ekman / (N * N)
+        para_a = 42.0 * math.sqrt(3.0) * diffusivity - 6.0
+        g_beta = self.Rayleigh * viscosity * diffusivity / N / (N * N)
+        return dict(tau=tauf, viscosity=viscosity, diffusivity=diffusivity, omega_rot=omega, para_a=para_a,
+                    g_beta=g_beta)
+
+    def params(self) -> LbmParams:
+        d = self.derived()
+        p = LbmParams()
+        p.nx, p.ny, p.nz = self.nx, self.ny, self.nz
+        p.collision, p.storage, p.thermal = LBM_MRT, STORE_F64, 1
+        p.block_x, p.block_y = self.block_x, self.block_y
+        p.thermal_axis = self.thermal_axis
+        p.tau, p.u_lid, p.rho0 = d["tau"], 0.0, 1.0
+        p.para_a, p.g_beta, p.omega_rot = d["para_a"], d["g_beta"], d["omega_rot"]
+        p.t_hot, p.t_cold, p.t_ref = self.Thot, self.Tcold, self.Tref
         return p
 
 
@@ -139,11 +228,13 @@ class Cavity:
     """One GPU's lid-driven cavity.  Arrays are returned as (nz, ny, nx) C-order
     numpy arrays, i.e. the reference's Fortran (nx, ny, nz) layout."""
 
-    def __init__(self, cfg: CavityConfig, stream: int | None = None):
+    def __init__(self, cfg, stream: int | None = None):
+        """cfg: CavityConfig (lid-driven) or ThermalConfig (buoyancy-driven),
+        or an LbmParams filled by the caller."""
         self.cfg = cfg
         self.lib = load_library()
         self._ctx = ctypes.c_void_p()
-        p = cfg.params()
+        p = cfg if isinstance(cfg, LbmParams) else cfg.params()
         self._call("lbm_create", ctypes.byref(p), ctypes.byref(self._ctx))
         if stream is not None:
             self._call("lbm_set_stream", self._ctx, ctypes.c_void_p(stream))
@@ -161,9 +252,10 @@ class Cavity:
     def initial(self):
         self._call("lbm_initial", self._ctx)
 
-    def load_fields(self, rho, u, v, w):
+    def load_fields(self, rho, u, v, w, T=None):
         arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(self.shape) for a in (rho, u, v, w)]
-        self._call("lbm_load_fields", self._ctx, *[_dp(a) for a in arrs])
+        t = None if T is None else np.ascontiguousarray(T, dtype=np.float64).reshape(self.shape)
+        self._call("lbm_load_fields", self._ctx, *[_dp(a) for a in arrs], _dp(t))
 
     def step(self, nsteps: int = 1):
         self._call("lbm_step", self._ctx, int(nsteps))
@@ -172,6 +264,45 @@ class Cavity:
         e = ctypes.c_double()
         self._call("lbm_check", self._ctx, ctypes.byref(e))
         return e.value
+
+    def check_thermal(self):
+        """(errorU, errorT) of the buoyancy check.F90."""
+        eu, et = ctypes.c_double(), ctypes.c_double()
+        self._call("lbm_check_thermal", self._ctx, ctypes.byref(eu), ctypes.byref(et))
+        return eu.value, et.value
+
+    def nusselt(self):
+        """(nu_vol, nu_wall) across the held walls."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._call("lbm_nusselt", self._ctx, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def temperature(self):
+        T = np.empty(self.shape, dtype=np.float64)
+        self._call("lbm_download_temperature", self._ctx, _dp(T))
+        return T
+
+    def check_sums(self):
+        """(e1, e2) of check.f90 on this subdomain, before the Allreduce."""
+        e1, e2 = ctypes.c_double(), ctypes.c_double()
+        self._call("lbm_check_sums", self._ctx, ctypes.byref(e1), ctypes.byref(e2))
+        return e1.value, e2.value
+
+    def halo_layout(self, region: int):
+        off, cnt = ctypes.c_longlong(), ctypes.c_longlong()
+        self._call("lbm_halo_layout", self._ctx, int(region), ctypes.byref(off), ctypes.byref(cnt))
+        return off.value, cnt.value
+
+    def halo_buffer_bytes(self) -> int:
+        n = ctypes.c_size_t()
+        self._call("lbm_halo_buffer_bytes", self._ctx, ctypes.byref(n))
+        return n.value
+
+    def halo_pack(self, dev_ptr: int):
+        self._call("lbm_halo_pack", self._ctx, ctypes.c_void_p(dev_ptr))
+
+    def halo_unpack(self, dev_ptr: int):
+        self._call("lbm_halo_unpack", self._ctx, ctypes.c_void_p(dev_ptr))
 
     def macro(self):
         out = [np.empty(self.shape, dtype=np.float64) for _ in range(4)]

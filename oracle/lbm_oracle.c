@@ -233,17 +233,30 @@ void oracle_cavity_collision(const oracle_cavity_params *p, const double *f,
 }
 
 /* streaming.f90:8-20: pull, f(a, x) = f_post(a, x - e_a).  Sources in the
- * halo hold nothing meaningful; bounceback overwrites every such value. */
+ * halo hold nothing meaningful on wall faces; bounceback overwrites every such
+ * value.  On periodic axes the source wraps to the opposite side, which is what
+ * the halo exchange of ex_sendrecv.f90 delivers when the neighbour across the
+ * face is the opposite face. */
+static int wrap(int s, int n, int periodic) {
+    if (!periodic) return s;
+    if (s < 0) return s + n;
+    if (s >= n) return s - n;
+    return s;
+}
+
 void oracle_cavity_streaming(const oracle_cavity_params *p, const double *fpost,
                              double *f) {
+    const int px = p->periodic_mask & 1, py = (p->periodic_mask >> 1) & 1, pz = (p->periodic_mask >> 2) & 1;
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < p->nz; ++k)
         for (int j = 0; j < p->ny; ++j)
             for (int i = 0; i < p->nx; ++i) {
                 size_t c = CELL(p, i, j, k);
-                for (int a = 0; a < 19; ++a)
-                    f[19 * c + a] =
-                        fpost[19 * HCELL(p, i + 1 - EX[a], j + 1 - EY[a], k + 1 - EZ[a]) + a];
+                for (int a = 0; a < 19; ++a) {
+                    int si = wrap(i - EX[a], p->nx, px), sj = wrap(j - EY[a], p->ny, py),
+                        sk = wrap(k - EZ[a], p->nz, pz);
+                    f[19 * c + a] = fpost[19 * HCELL(p, si + 1, sj + 1, sk + 1) + a];
+                }
             }
 }
 
@@ -262,6 +275,7 @@ void oracle_cavity_bounceback(const oracle_cavity_params *p, const double *fpost
     /* incoming populations per wall: those whose e points into the domain */
     for (int wall = 0; wall < 6; ++wall) {
         int axis = wall / 2, hi = wall % 2;
+        if ((p->periodic_mask >> axis) & 1) continue;  /* no wall on a periodic axis */
         int n0 = axis == 0 ? ny : nx, n1 = axis == 2 ? ny : nz;
         for (int b = 0; b < n1; ++b)
             for (int a0 = 0; a0 < n0; ++a0) {

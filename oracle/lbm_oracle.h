@@ -26,6 +26,9 @@ typedef struct {
     double u_lid;        /* U0 */
     double rho0;
     int meq12_rho;       /* 0: cavity meq(12) without rho (collision.f90:85) */
+    int periodic_mask;   /* bit 0/1/2: x/y/z faces exchange with the opposite face
+                            (the halo a periodic Cartesian exchange would deliver)
+                            instead of bouncing back; 0 = the reference cavity */
 } oracle_cavity_params;
 
 /* Whole-run driver: reference initial, then nsteps of
@@ -65,6 +68,33 @@ double oracle_cavity_check(const oracle_cavity_params *p, const double *u,
  * inverse via row norms (M^-1 = M^T diag(1/|row|^2)).  Exposed so tests can
  * check M M^-1 = I and the equilibrium moments M feq = meq. */
 void oracle_d3q19_basis(double M[19][19], double Minv[19][19]);
+
+/* ---- buoyancy-driven cavity: D3Q19 + Guo forcing, D3Q7 temperature ---- */
+typedef struct {
+    int nx, ny, nz;
+    int tconst_axis;     /* walls held at t_hot (low side) / t_cold (high side):
+                            1 = y (LeftRightWallsConstT), 2 = z (TopBottomPlatesConstT),
+                            0 = x, -1 = none; the other walls are adiabatic */
+    double tau;          /* tauf (module.F90:19) */
+    double para_a;       /* paraA = 42 sqrt(3) kappa - 6 (module.F90:27) */
+    double g_beta;       /* gBeta (module.F90:28-29) */
+    double omega_rot;    /* Coriolis rate omegaRatating (module.F90:25); 0 = off */
+    double t_hot, t_cold, t_ref;
+} oracle_thermal_params;
+
+/* Reference initial T (or the given fields when from_fields), then nsteps of
+ * collision -> streaming -> bounceback -> collisionT -> streamingT ->
+ * bouncebackT -> macro -> macroT (bouyancy3d_mpi.F90:222-248).  errs receives
+ * (errorU, errorT) pairs of check.F90 every check_every steps. */
+int oracle_thermal_run(const oracle_thermal_params *p, int nsteps, int check_every, double *rho,
+                       double *u, double *v, double *w, double *T, int from_fields, double *errs,
+                       int nerr_max, int nthreads);
+void oracle_thermal_initial_T(const oracle_thermal_params *p, double *T);
+void oracle_guo_source_moments(double u, double v, double w, double Fx, double Fy, double Fz,
+                               double *S);
+void oracle_d3q7_basis(double N[7][7], double Ninv[7][7]);
+void oracle_thermal_nusselt(const oracle_thermal_params *p, const double *ua, const double *T,
+                            double kappa, double *nu_vol, double *nu_wall);
 
 /* Wall-clock seconds spent inside the step loop of the last run. */
 double oracle_last_loop_seconds(void);

@@ -30,6 +30,7 @@ class CavityParams(ctypes.Structure):
         ("u_lid", ctypes.c_double),
         ("rho0", ctypes.c_double),
         ("meq12_rho", ctypes.c_int),
+        ("periodic_mask", ctypes.c_int),
     ]
 
 
@@ -66,8 +67,9 @@ def _dp(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-def cavity_params(nx, ny, nz, tau, u_lid=0.1, collision=MRT, rho0=1.0, meq12_rho=0) -> CavityParams:
-    return CavityParams(nx, ny, nz, collision, tau, u_lid, rho0, meq12_rho)
+def cavity_params(nx, ny, nz, tau, u_lid=0.1, collision=MRT, rho0=1.0, meq12_rho=0,
+                  periodic_mask=0) -> CavityParams:
+    return CavityParams(nx, ny, nz, collision, tau, u_lid, rho0, meq12_rho, periodic_mask)
 
 
 def fields(nx, ny, nz):
@@ -117,3 +119,102 @@ def run_reference_cavity(nx, ny, nz, nsteps, outfile):
     assert body.size == 4 * n, (body.size, n)
     rho, u, v, w = (body[i * n:(i + 1) * n].reshape(nz, ny, nx) for i in range(4))
     return dict(tauf=head[0], U0=head[1], rho0=head[2]), rho, u, v, w
+
+
+# ---- buoyancy-driven cavity (D3Q19 + Guo forcing, D3Q7 temperature) ----------
+REF_THERMAL = os.path.join(HERE, "_ref", "ref_thermal")
+
+
+class ThermalParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("tconst_axis", ctypes.c_int),
+        ("tau", ctypes.c_double), ("para_a", ctypes.c_double), ("g_beta", ctypes.c_double),
+        ("omega_rot", ctypes.c_double), ("t_hot", ctypes.c_double), ("t_cold", ctypes.c_double),
+        ("t_ref", ctypes.c_double),
+    ]
+
+
+def thermal_derived(N, Rayleigh, Prandtl, Mach=0.1, Ekman=None):
+    """The parameter statements of module.F90:19-29, computed in the same order."""
+    import math
+    tauf = 0.5 + Mach * float(N) * math.sqrt(3.0 * Prandtl / Rayleigh)
+    viscosity = (tauf - 0.5) / 3.0
+    diffusivity = viscosity / Prandtl
+    omega_rot = 0.0 if Ekman is None else viscosity / 2.0 / Ekman / float(N * N)
+    para_a = 42.0 * math.sqrt(3.0) * diffusivity - 6.0
+    g_beta1 = Rayleigh * viscosity * diffusivity / float(N)
+    g_beta = g_beta1 / float(N * N)
+    return dict(tau=tauf, viscosity=viscosity, diffusivity=diffusivity, omega_rot=omega_rot, para_a=para_a,
+                g_beta=g_beta)
+
+
+def thermal_params(nx, ny, nz, tau, para_a, g_beta, omega_rot=0.0, t_hot=0.5, t_cold=-0.5, t_ref=0.0,
+                   tconst_axis=1) -> ThermalParams:
+    return ThermalParams(nx, ny, nz, tconst_axis, tau, para_a, g_beta, omega_rot, t_hot, t_cold, t_ref)
+
+
+def _thermal_lib():
+    L = lib()
+    if not getattr(L, "_thermal_ready", False):
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.oracle_thermal_run.argtypes = [ctypes.POINTER(ThermalParams), ctypes.c_int, ctypes.c_int, dp, dp, dp,
+                                         dp, dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int]
+        L.oracle_thermal_run.restype = ctypes.c_int
+        L.oracle_guo_source_moments.argtypes = [ctypes.c_double] * 6 + [dp]
+        L.oracle_d3q7_basis.argtypes = [dp, dp]
+        L.oracle_thermal_nusselt.argtypes = [ctypes.POINTER(ThermalParams), dp, dp, ctypes.c_double, dp, dp]
+        L._thermal_ready = True
+    return L
+
+
+def thermal_run(p: ThermalParams, nsteps: int, fields=None, check_every: int = 0, nthreads: int = 0):
+    """Reference initial (or `fields` = (rho, u, v, w, T)) + nsteps steps.
+    Returns (rho, u, v, w, T, errs) with errs an (n, 2) array of (errorU, errorT)."""
+    shape = (p.nz, p.ny, p.nx)
+    if fields is None:
+        arrs = [np.zeros(shape) for _ in range(5)]
+        from_fields = 0
+    else:
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(shape).copy() for a in fields]
+        from_fields = 1
+    nmax = max(1, nsteps // check_every) if check_every > 0 else 1
+    errs = np.zeros(2 * nmax)
+    r = _thermal_lib().oracle_thermal_run(ctypes.byref(p), nsteps, check_every, *[_dp(a) for a in arrs],
+                                          from_fields, _dp(errs), nmax, nthreads)
+    if r < 0:
+        raise MemoryError("oracle_thermal_run failed to allocate")
+    return (*arrs, errs[:2 * r].reshape(r, 2))
+
+
+def guo_source_moments(u, v, w, Fx, Fy, Fz):
+    S = np.zeros(19)
+    _thermal_lib().oracle_guo_source_moments(u, v, w, Fx, Fy, Fz, _dp(S))
+    return S
+
+
+def d3q7_basis():
+    N = np.zeros((7, 7)); Ninv = np.zeros((7, 7))
+    _thermal_lib().oracle_d3q7_basis(_dp(N), _dp(Ninv))
+    return N, Ninv
+
+
+def thermal_nusselt(p: ThermalParams, ua, T, kappa):
+    a = np.ascontiguousarray(ua, dtype=np.float64)
+    t = np.ascontiguousarray(T, dtype=np.float64)
+    nv, nw = ctypes.c_double(), ctypes.c_double()
+    _thermal_lib().oracle_thermal_nusselt(ctypes.byref(p), _dp(a), _dp(t), kappa, ctypes.byref(nv), ctypes.byref(nw))
+    return nv.value, nw.value
+
+
+def run_reference_thermal(nx, ny, nz, nsteps, outfile):
+    """Run oracle/_ref/ref_thermal (the reference's own buoyancy subroutines) and
+    read back its parameters and rho, u, v, w, T."""
+    subprocess.run([REF_THERMAL, str(nx), str(ny), str(nz), str(nsteps), outfile], check=True,
+                   stdout=subprocess.DEVNULL)
+    raw = np.fromfile(outfile, dtype=np.float64)
+    head, body = raw[:8], raw[8:]
+    n = nx * ny * nz
+    assert body.size == 5 * n, (body.size, n)
+    keys = ("tauf", "paraA", "gBeta", "omegaRatating", "Thot", "Tcold", "Tref", "diffusivity")
+    fields = [body[i * n:(i + 1) * n].reshape(nz, ny, nx) for i in range(5)]
+    return dict(zip(keys, head)), fields

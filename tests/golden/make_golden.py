@@ -1,4 +1,5 @@
-"""Generate the golden fixtures of the D3Q19 cavity from the reference itself.
+"""Generate the golden fixtures of the D3Q19 cavity and the buoyancy cavity
+from the reference itself.
 
 Runs oracle/_ref/ref_cavity -- the reference's own Fortran subroutines
 (MPI/Lid_driven_cavity/fortran/3d/mpi_3d_blocked/{commondata,initial,
@@ -9,6 +10,12 @@ container that has /root/reference; the fixtures are committed.
 The reference's tauf, U0 and Reynolds are compile-time parameters
 (commondata.f90:4-9: Re = 1000, U0 = 0.1, N = 65, tauf = 0.5195), so the
 fixtures are MRT at that tau on small grids of varying shape.
+
+Buoyancy fixtures run oracle/_ref/ref_thermal, the reference's own
+module/initial/evolution_f/evolution_g.F90 built with its Makefile's macros
+(no-slip walls, y walls held at Thot = 1 / Tcold = 0, other walls adiabatic),
+at its compile-time parameters (module.F90: Ra = 1e6, Pr = 0.71, Ma = 0.1,
+N = 51, Ekman = 0.001 so the Coriolis force is on).
 
 usage: python tests/golden/make_golden.py
 """
@@ -35,6 +42,14 @@ CASES = [
     (20, 17, 13, 1000),
 ]
 
+THERMAL_CASES = [
+    (8, 9, 10, 1),
+    (8, 9, 10, 10),
+    (12, 10, 9, 300),
+    (16, 16, 16, 500),
+    (20, 17, 13, 1000),
+]
+
 
 def main():
     subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle", "ref", "Makefile")], check=True)
@@ -44,6 +59,12 @@ def main():
             name = os.path.join(HERE, f"cavity_mrt_{nx}x{ny}x{nz}_s{ns}.npz")
             np.savez_compressed(name, nx=nx, ny=ny, nz=nz, steps=ns, tauf=hdr["tauf"], U0=hdr["U0"],
                                 rho0=hdr["rho0"], rho=rho, u=u, v=v, w=w)
+            print("wrote", os.path.relpath(name, ROOT))
+        for nx, ny, nz, ns in THERMAL_CASES:
+            hdr, (rho, u, v, w, T) = O.run_reference_thermal(nx, ny, nz, ns, os.path.join(td, "out.bin"))
+            name = os.path.join(HERE, f"thermal_{nx}x{ny}x{nz}_s{ns}.npz")
+            np.savez_compressed(name, nx=nx, ny=ny, nz=nz, steps=ns, rho=rho, u=u, v=v, w=w, T=T,
+                                **{k: float(val) for k, val in hdr.items()})
             print("wrote", os.path.relpath(name, ROOT))
 
 

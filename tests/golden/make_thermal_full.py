@@ -1,0 +1,51 @@
+"""Full-size buoyancy-cavity fixture, written by the oracle (not the reference).
+
+The reference's buoyancy parameters are compile-time (module.F90: Ra = 1e6,
+N = 51), so it cannot run the 128^3, Ra = 1e5 case; the oracle -- pinned to
+the reference by the thermal_*.npz fixtures and closed forms -- runs it once
+here (about 10 minutes on 8 cores) and the summary is committed:
+Nusselt numbers, residuals, mid-plane profiles and a strided subsample of
+the fields after 20000 steps.
+
+usage: python tests/golden/make_thermal_full.py [steps] [nthreads]
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from mglc_amd.lbm import ThermalConfig, buoyancy_initial_temperature  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+N, STEPS, SEED, STRIDE = 128, 20000, 2024, 8
+
+
+def main():
+    steps = int(sys.argv[1]) if len(sys.argv) > 1 else STEPS
+    threads = int(sys.argv[2]) if len(sys.argv) > 2 else 8
+    cfg = ThermalConfig(N, N, N, Rayleigh=1e5, Prandtl=0.71, Thot=0.5, Tcold=-0.5, thermal_axis=1)
+    d = cfg.derived()
+    T0 = buoyancy_initial_temperature(N, N, N, seed=SEED)
+    z = np.zeros_like(T0)
+    p = O.thermal_params(N, N, N, d["tau"], d["para_a"], d["g_beta"], 0.0, 0.5, -0.5, 0.0, tconst_axis=1)
+    t0 = time.time()
+    rho, u, v, w, T, errs = O.thermal_run(p, steps, fields=(np.ones_like(T0), z, z, z, T0), check_every=2000,
+                                          nthreads=threads)
+    print(f"oracle {N}^3 x {steps} steps: {time.time() - t0:.0f} s")
+    nu = O.thermal_nusselt(p, v, T, d["diffusivity"])
+    h = N // 2
+    s = slice(STRIDE // 2, None, STRIDE)
+    name = os.path.join(HERE, f"thermal_full_{N}_s{steps}.npz")
+    np.savez_compressed(name, N=N, steps=steps, seed=SEED, stride=STRIDE, nu_vol=nu[0], nu_wall=nu[1],
+                        errs=errs, T_mid=T[h, :, h], v_mid=v[h, :, h], w_mid=w[h, :, h],
+                        rho_sub=rho[s, s, s], u_sub=u[s, s, s], v_sub=v[s, s, s], w_sub=w[s, s, s],
+                        T_sub=T[s, s, s], mass=float(rho.sum()))
+    print("wrote", os.path.relpath(name, ROOT), "Nu", nu)
+
+
+if __name__ == "__main__":
+    main()

@@ -35,6 +35,14 @@ def test_create_rejects_bad_tau_and_grid(hip_lib):
     assert hip_lib.lbm_create(ctypes.byref(p), ctypes.byref(ctx)) == 1  # LBM_ERR_ARG
     p = lbm.CavityConfig(8, 8, 8, block_x=48).params()
     assert hip_lib.lbm_create(ctypes.byref(p), ctypes.byref(ctx)) == 1
+    p = lbm.ThermalConfig(8, 8, 8).params()
+    p.storage = 1
+    assert hip_lib.lbm_create(ctypes.byref(p), ctypes.byref(ctx)) == 7  # LBM_ERR_UNSUPPORTED
+    p = lbm.ThermalConfig(8, 8, 8).params()
+    p.u_lid = 0.1
+    assert hip_lib.lbm_create(ctypes.byref(p), ctypes.byref(ctx)) == 1
+    p = lbm.CavityConfig(8, 8, 8, halo_faces=64).params()
+    assert hip_lib.lbm_create(ctypes.byref(p), ctypes.byref(ctx)) == 1
     assert not ctx.value
     assert hip_lib.lbm_step(None, 1) == 1
     assert hip_lib.lbm_status_string(3) == b"residual is NaN"
@@ -44,6 +52,7 @@ def test_struct_layout_matches_header():
     """ctypes mirror of lbm_params / lbm_info has the C sizes (ints then doubles)."""
     from mglc_amd import lbm
     assert ctypes.sizeof(lbm.LbmParams) == 16 * 4 + 11 * 8
+    assert lbm.LbmParams.tau.offset == 64 and lbm.LbmParams.t_ref.offset == 64 + 8 * 8
     assert ctypes.sizeof(lbm.LbmInfo) == 7 * 8 + 8 + 2 * 4
 
 
