@@ -189,6 +189,46 @@ int lbm_download_macro(lbm_ctx *ctx, double *rho, double *u, double *v, double *
 int lbm_synchronize(lbm_ctx *ctx);
 
 int lbm_get_info(const lbm_ctx *ctx, lbm_info *info);
+
+/* ---- batched cavities ----------------------------------------------------
+ * Many independent lid-driven cavities of uneven sizes, Re and U0 stepped
+ * together: one persistent kernel walks 64-cell tiles of every instance
+ * through a per-instance offset table; whole 100-step blocks are replayed
+ * from a captured hipGraph; the persistent launch shape is picked by
+ * lbm_batch_autotune.  Each instance reproduces a single-context run of the
+ * same parameters exactly.  No reference counterpart: the reference runs one
+ * cavity per program (mpi_3d_blocked/main.f90). */
+typedef struct lbm_batch lbm_batch;
+
+typedef struct lbm_batch_info {
+    long long instances, total_cells, total_tiles, steps_done, device_bytes;
+    double bytes_per_lup;       /* 2*19*sizeof(store) */
+    int threads, blocks;        /* persistent launch shape in use */
+    int graph_steps;            /* steps per captured graph */
+    void *stream;               /* the batch's own hipStream_t */
+} lbm_batch_info;
+
+/* params[m] gives instance m's nx, ny, nz, tau and u_lid; storage, collision
+ * and meq12_rho must be the same for all (instance 0's). */
+int lbm_batch_create(const lbm_params *params, int count, lbm_batch **batch);
+int lbm_batch_destroy(lbm_batch *batch);
+/* Reference initial state (initial.f90:46-73) in every instance. */
+int lbm_batch_initial(lbm_batch *batch);
+/* nsteps steps of every instance: the persistent kernel, graph-replayed in
+ * blocks of 100 steps.  Macro fields of the last step are kept. */
+int lbm_batch_step(lbm_batch *batch, int nsteps);
+/* The same steps with one single-domain launch per instance per step (the
+ * instances launched one at a time): the baseline the batch is measured
+ * against. */
+int lbm_batch_step_each(lbm_batch *batch, int nsteps);
+/* Time trial_steps steps at each candidate launch shape (64/128/256 threads,
+ * 4..64 waves per CU), keep the fastest, and restart from the initial state. */
+int lbm_batch_autotune(lbm_batch *batch, int trial_steps, int *threads, int *blocks);
+int lbm_batch_set_launch(lbm_batch *batch, int threads, int blocks);
+int lbm_batch_synchronize(lbm_batch *batch);
+int lbm_batch_download_macro(lbm_batch *batch, int instance, double *rho, double *u, double *v,
+                             double *w);
+int lbm_batch_get_info(const lbm_batch *batch, lbm_batch_info *info);
 const char *lbm_status_string(int status);
 const char *lbm_last_error(void);
 

@@ -29,158 +29,22 @@
 #include "collide.hpp"
 #include "context.hpp"
 #include "halo.hpp"
+#include "step.hpp"
 
 using namespace lbm;
 
 namespace {
-
-struct StepArgs {
-    const void* src;
-    void* dst;
-    int nx, ny, nz, px;
-    long long pxy;      // px * ny
-    long long ps;       // population stride (elements)
-    double u_lid;
-    double snu;         // 1 / tau
-    double sq;          // 8 (2 tau - 1) / (8 tau - 1)
-    float scale, inv_scale;
-    int meq12_rho;
-    double* lid_rho;    // nx * ny: rho of the top layer from the last macro
-    double* rho;        // compact nx*ny*nz macro output (written when MACRO)
-    double* u;
-    double* v;
-    double* w;
-    int halo_faces;     // bit f: face f reads the halo instead of bouncing back
-    const void* halo;   // incoming regions, packed layout (HALO kernels only)
-    long long hoff[NREGION];    // element offset of each region in the packed layout
-    long long hcount[NREGION];  // elements in each region
-};
-
-enum { COLL_MRT = 0, COLL_BGK = 1 };
-
-template <class S, int COLL>
-__device__ __forceinline__ void collide(typename S::Real f[Q],
-                                        const Moments<typename S::Real, S::SHIFT>& q,
-                                        const StepArgs& a) {
-    using R = typename S::Real;
-    if constexpr (COLL == COLL_BGK)
-        collide_bgk<R, S::SHIFT>(f, q, R(a.snu));
-    else
-        collide_mrt<R, S::SHIFT>(f, q, R(a.snu), R(a.sq), a.meq12_rho != 0);
-}
-
-// Value pulled from outside the domain for population al at cell (i, j, k):
-// from the halo when every face the pull crosses borders a neighbour
-// subdomain, else halfway bounce-back from this cell's opposite population
-// (f(a, x) = f_post(abar, x), bounce_back.f90:6-83), with the moving-lid term
-// when the pull crosses the top wall.  A wall crossed anywhere wins, as the
-// reference's bounceback overwrites what streaming pulled from its halo.
-template <class S, int al, bool HALO>
-__device__ __forceinline__ typename S::Real pull_outside(const StepArgs& a, const typename S::T* src,
-                                                         int i, int j, int k, int c, bool top,
-                                                         double rho_lid) {
-    using T = typename S::T;
-    using R = typename S::Real;
-    constexpr int ex = EX[al], ey = EY[al], ez = EZ[al];
-    bool lid = top;  // the z+ face is the moving lid when it is a wall
-    if constexpr (HALO) {
-        bool cx = false, cy = false, cz = false;
-        if constexpr (ex != 0) cx = (unsigned)(i - ex) >= (unsigned)a.nx;
-        if constexpr (ey != 0) cy = (unsigned)(j - ey) >= (unsigned)a.ny;
-        if constexpr (ez != 0) cz = (unsigned)(k - ez) >= (unsigned)a.nz;
-        constexpr int fx = ex != 0 ? entry_face(al, 0) : 0;
-        constexpr int fy = ey != 0 ? entry_face(al, 1) : 0;
-        constexpr int fz = ez != 0 ? entry_face(al, 2) : 0;
-        const bool wall = (cx && !((a.halo_faces >> fx) & 1)) || (cy && !((a.halo_faces >> fy) & 1)) ||
-                          (cz && !((a.halo_faces >> fz) & 1));
-        if (!wall) {
-            const T* h = static_cast<const T*>(a.halo);
-            long long idx;
-            if (int(cx) + int(cy) + int(cz) == 2) {
-                constexpr int free_axis = al >= 7 ? edge_free_axis(al) : 0;
-                const int t = free_axis == 0 ? i : (free_axis == 1 ? j : k);
-                idx = a.hoff[al >= 7 ? edge_region(al) : 0] + t;
-            } else if (cx) {
-                idx = a.hoff[fx] + (long long)cross_slot(fx, al, -1) * a.ny * a.nz + (j - ey) +
-                      (long long)a.ny * (k - ez);
-            } else if (cy) {
-                idx = a.hoff[fy] + (long long)cross_slot(fy, al, -1) * a.nx * a.nz + (i - ex) +
-                      (long long)a.nx * (k - ez);
-            } else {
-                idx = a.hoff[fz] + (long long)cross_slot(fz, al, -1) * a.nx * a.ny + (i - ex) +
-                      (long long)a.nx * (j - ey);
-            }
-            return S::ld(h + idx, a.inv_scale);
-        }
-        lid = top && !((a.halo_faces >> LBM_FACE_ZP) & 1);
-    }
-    R val = S::ld(src + (long long)OPP[al] * a.ps + c, a.inv_scale);
-    if constexpr (ez == -1 && ex != 0) {
-        // moving lid (bounce_back.f90:77-78): f(14) = f_post(11) - rho/6 U0,
-        // f(13) = f_post(12) - rho/6 (-U0); the lid is applied last, so it
-        // wins over the side walls on these two populations.
-        if (lid) {
-            const double t = rho_lid / 6.0 * a.u_lid;
-            val = (ex > 0) ? R(double(val) + t) : R(double(val) - t);
-        }
-    }
-    return val;
-}
 
 // One lattice update per thread.  Block = (bx, by) with bx a multiple of 64:
 // one wavefront per x-row segment of 64 cells, so every population load and
 // store of a wave is 64 consecutive elements.
 template <class S, int COLL, bool MACRO, bool HALO>
 __global__ __launch_bounds__(256) void k_cavity_step(const StepArgs a) {
-    using T = typename S::T;
-    using R = typename S::Real;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= a.nx || j >= a.ny) return;
-    const int c = i + a.px * (j + a.ny * k);
-    const T* __restrict__ src = static_cast<const T*>(a.src);
-    T* __restrict__ dst = static_cast<T*>(a.dst);
-    const bool top = (k == a.nz - 1);
-    const int lid_idx = i + a.nx * j;
-    double rho_lid = 0.0;
-    if (top) rho_lid = a.lid_rho[lid_idx];
-
-    R f[Q];
-    sfor<Q>([&](auto A) {
-        constexpr int al = decltype(A)::value;
-        constexpr int ex = EX[al], ey = EY[al], ez = EZ[al];
-        const T* base = src + (long long)al * a.ps;
-        if constexpr (ex == 0 && ey == 0 && ez == 0) {
-            f[al] = S::ld(base + c, a.inv_scale);
-        } else {
-            bool in = true;
-            if constexpr (ex != 0) in = in && (unsigned)(i - ex) < (unsigned)a.nx;
-            if constexpr (ey != 0) in = in && (unsigned)(j - ey) < (unsigned)a.ny;
-            if constexpr (ez != 0) in = in && (unsigned)(k - ez) < (unsigned)a.nz;
-            if (in) {
-                const long long off = (long long)ex + (long long)ey * a.px + (long long)ez * a.pxy;
-                f[al] = S::ld(base + (c - off), a.inv_scale);
-            } else {
-                f[al] = pull_outside<S, al, HALO>(a, src, i, j, k, c, top, rho_lid);
-            }
-        }
-    });
-
-    const Moments<R, S::SHIFT> q = moments<R, S::SHIFT>(f);
-    if (top) a.lid_rho[lid_idx] = S::SHIFT ? 1.0 + double(q.drho) : double(q.rho);
-    if constexpr (MACRO) {
-        const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
-        a.rho[m] = S::SHIFT ? 1.0 + double(q.drho) : double(q.rho);
-        a.u[m] = double(q.u);
-        a.v[m] = double(q.v);
-        a.w[m] = double(q.w);
-    }
-    collide<S, COLL>(f, q, a);
-    sfor<Q>([&](auto A) {
-        constexpr int al = decltype(A)::value;
-        S::st(dst + (long long)al * a.ps + c, f[al], a.scale);
-    });
+    cavity_cell<S, COLL, MACRO, HALO>(a, i, j, k);
 }
 
 struct LeavingTable {
@@ -285,7 +149,7 @@ __global__ __launch_bounds__(256) void k_cavity_init(const StepArgs a, const dou
     T* __restrict__ dst = static_cast<T*>(a.dst);
     sfor<Q>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        S::st(dst + (long long)al * a.ps + c, f[al], a.scale);
+        S::st(dst + (long long)al * a.ps + c, f[al], a.scale, store_random<S>(c, al, a.seed));
     });
 }
 
@@ -515,6 +379,21 @@ void release(lbm_ctx* c) {
 namespace {
 thread_local char g_err[512] = "";
 }  // namespace
+
+int lbm_launch_cavity_step(const StepArgs& a, int storage, int collision, bool macro, hipStream_t stream) {
+    const int coll = collision == LBM_BGK ? 1 : 0;
+    StepFn fn;
+    switch (storage) {
+    case LBM_STORE_F32: fn = Table<StoreF32>::step[coll][macro][0]; break;
+    case LBM_STORE_F16: fn = Table<StoreF16>::step[coll][macro][0]; break;
+    default: fn = Table<StoreF64>::step[coll][macro][0];
+    }
+    const int bx = a.nx >= 256 ? 256 : (a.nx + 63) / 64 * 64;
+    const dim3 block(bx, 256 / bx, 1);
+    const dim3 grid((a.nx + block.x - 1) / block.x, (a.ny + block.y - 1) / block.y, a.nz);
+    hipLaunchKernelGGL(fn, grid, block, 0, stream, a);
+    return lbm_check_launch();
+}
 
 int lbm_fail(int code, const char* fmt, ...) {
     va_list ap;
@@ -771,6 +650,7 @@ int lbm_step(lbm_ctx* c, int nsteps) {
         }
         a.src = c->buf[c->cur];
         a.dst = c->buf[1 - c->cur];
+        a.seed = (unsigned)(c->itc + s + 1);
         hipLaunchKernelGGL(step_fn(c, s == nsteps - 1), grid, c->block, 0, c->stream, a);
         c->cur = 1 - c->cur;
     }

@@ -43,38 +43,77 @@ __device__ __forceinline__ R mac(R acc, R x) {
 // Stores are non-temporal: the next step reads the written lattice only after
 // the whole sweep, so caching the lines on the way out buys nothing (measured
 // 2-4% faster at 256^3).
+// Every st() takes a 32-bit random word; only the FP16 format uses it.
 struct StoreF64 {
     using T = double;
     using Real = double;
     static constexpr bool SHIFT = false;
+    static constexpr bool RANDOM = false;
     static constexpr int BYTES = 8;
     __device__ __forceinline__ static Real ld(const T* p, float) { return *p; }
-    __device__ __forceinline__ static void st(T* p, Real v, float) { __builtin_nontemporal_store(v, p); }
+    __device__ __forceinline__ static void st(T* p, Real v, float, unsigned) { __builtin_nontemporal_store(v, p); }
 };
 
 struct StoreF32 {
     using T = float;
     using Real = float;
     static constexpr bool SHIFT = true;
+    static constexpr bool RANDOM = false;
     static constexpr int BYTES = 4;
     __device__ __forceinline__ static Real ld(const T* p, float) { return *p; }
-    __device__ __forceinline__ static void st(T* p, Real v, float) { __builtin_nontemporal_store(v, p); }
+    __device__ __forceinline__ static void st(T* p, Real v, float, unsigned) { __builtin_nontemporal_store(v, p); }
 };
 
+// FP32 -> IEEE binary16 with stochastic rounding: the 13 mantissa bits the
+// conversion drops are replaced by rounding up with probability equal to
+// their value (add a uniform 13-bit word to the magnitude, truncate).  The
+// result is exact in binary16 for the normal range; the final conversion
+// only rounds values in binary16's subnormal range.  Rounding to nearest
+// would round a slowly varying population the same way every step -- a
+// constant spurious force that the flow integrates over its long viscous
+// time; stochastic rounding makes that error unbiased.
+__device__ __forceinline__ _Float16 to_half_stochastic(float x, unsigned rnd) {
+    unsigned b = __float_as_uint(x);
+    unsigned mag = b & 0x7fffffffu;
+    if (mag < 0x7f800000u) mag = (mag + (rnd & 0x1fffu)) & ~0x1fffu;
+    return static_cast<_Float16>(__uint_as_float((b & 0x80000000u) | mag));
+}
+
+// Avalanche hash of a 32-bit word (the "lowbias32" integer mixer), used to
+// draw the per-store random words from (cell, population, step).
+__device__ __forceinline__ unsigned mix32(unsigned x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
 // IEEE binary16 holding (f - w) * 2^k.  The scale is an exact power of two,
-// so scaling adds no rounding; the store rounds to nearest even.
+// so scaling adds no rounding; the store rounds stochastically.
 struct StoreF16 {
     using T = _Float16;
     using Real = float;
     static constexpr bool SHIFT = true;
+    static constexpr bool RANDOM = true;
     static constexpr int BYTES = 2;
     __device__ __forceinline__ static Real ld(const T* p, float inv_scale) {
         return static_cast<float>(*p) * inv_scale;
     }
-    __device__ __forceinline__ static void st(T* p, Real v, float scale) {
-        __builtin_nontemporal_store(static_cast<_Float16>(v * scale), p);
+    __device__ __forceinline__ static void st(T* p, Real v, float scale, unsigned rnd) {
+        __builtin_nontemporal_store(to_half_stochastic(v * scale, rnd), p);
     }
 };
+
+// Random word for the store of population a of cell c at step `seed`.
+template <class S>
+__device__ __forceinline__ unsigned store_random(long long c, int a, unsigned seed) {
+    if constexpr (S::RANDOM)
+        return mix32((unsigned)c * 19u + (unsigned)a + seed * 0x9E3779B9u);
+    else
+        return 0u;
+}
 
 // ---- macroscopic moments (macro.f90:6-25) ---------------------------------------
 // Sums run a = 0..18 in order, as the reference accumulates them.

@@ -32,6 +32,9 @@ EXPORTED_SYMBOLS = (
     "lbm_check", "lbm_download_macro", "lbm_synchronize", "lbm_get_info", "lbm_status_string",
     "lbm_last_error", "lbm_check_sums", "lbm_halo_layout", "lbm_halo_buffer_bytes", "lbm_halo_pack",
     "lbm_halo_unpack", "lbm_check_thermal", "lbm_nusselt", "lbm_download_temperature",
+    "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_initial", "lbm_batch_step", "lbm_batch_step_each",
+    "lbm_batch_autotune", "lbm_batch_set_launch", "lbm_batch_synchronize", "lbm_batch_download_macro",
+    "lbm_batch_get_info",
 )
 
 FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = range(6)
@@ -59,6 +62,15 @@ class LbmInfo(ctypes.Structure):
         ("pitch_x", ctypes.c_longlong), ("pop_stride", ctypes.c_longlong),
         ("steps_done", ctypes.c_longlong), ("device_bytes", ctypes.c_longlong),
         ("bytes_per_lup", ctypes.c_double), ("storage", ctypes.c_int), ("collision", ctypes.c_int),
+    ]
+
+
+class LbmBatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("instances", ctypes.c_longlong), ("total_cells", ctypes.c_longlong), ("total_tiles", ctypes.c_longlong),
+        ("steps_done", ctypes.c_longlong), ("device_bytes", ctypes.c_longlong), ("bytes_per_lup", ctypes.c_double),
+        ("threads", ctypes.c_int), ("blocks", ctypes.c_int), ("graph_steps", ctypes.c_int),
+        ("stream", ctypes.c_void_p),
     ]
 
 
@@ -98,6 +110,16 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lbm_halo_buffer_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
     lib.lbm_halo_pack.argtypes = [vp, vp]
     lib.lbm_halo_unpack.argtypes = [vp, vp]
+    lib.lbm_batch_create.argtypes = [ctypes.POINTER(LbmParams), ip, ctypes.POINTER(vp)]
+    lib.lbm_batch_destroy.argtypes = [vp]
+    lib.lbm_batch_initial.argtypes = [vp]
+    lib.lbm_batch_step.argtypes = [vp, ip]
+    lib.lbm_batch_step_each.argtypes = [vp, ip]
+    lib.lbm_batch_autotune.argtypes = [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(ip)]
+    lib.lbm_batch_set_launch.argtypes = [vp, ip, ip]
+    lib.lbm_batch_synchronize.argtypes = [vp]
+    lib.lbm_batch_download_macro.argtypes = [vp, ip, dp, dp, dp, dp]
+    lib.lbm_batch_get_info.argtypes = [vp, ctypes.POINTER(LbmBatchInfo)]
     lib.lbm_status_string.argtypes = [ip]
     lib.lbm_status_string.restype = ctypes.c_char_p
     lib.lbm_last_error.restype = ctypes.c_char_p
@@ -321,6 +343,84 @@ class Cavity:
         if self._ctx:
             self.lib.lbm_destroy(self._ctx)
             self._ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_configs(count: int = 64, seed: int = 12345, sizes=(32, 48, 64, 80, 96), storage: str = "f32",
+                  collision: str = "mrt"):
+    """The batched-cavity case: `count` cavities with N drawn from `sizes`, Re
+    uniform in [100, 1000] and U0 uniform in [0.05, 0.1], from a seeded numpy
+    generator (PCG64)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.choice(np.asarray(sizes), size=count)
+    res = rng.uniform(100.0, 1000.0, size=count)
+    us = rng.uniform(0.05, 0.1, size=count)
+    return [CavityConfig(int(n), int(n), int(n), Reynolds=float(r), U0=float(u), storage=storage, collision=collision)
+            for n, r, u in zip(ns, res, us)]
+
+
+class Batch:
+    """Independent lid-driven cavities stepped together (lbm_batch_*)."""
+
+    def __init__(self, cfgs):
+        self.cfgs = list(cfgs)
+        self.lib = load_library()
+        arr = (LbmParams * len(self.cfgs))(*[c.params() for c in self.cfgs])
+        self._b = ctypes.c_void_p()
+        self._call("lbm_batch_create", arr, len(self.cfgs), ctypes.byref(self._b))
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, name)(*args)
+        if rc != LBM_OK:
+            raise LbmError(rc, name, self.lib.lbm_last_error().decode())
+        return rc
+
+    def initial(self):
+        self._call("lbm_batch_initial", self._b)
+
+    def step(self, nsteps: int):
+        self._call("lbm_batch_step", self._b, int(nsteps))
+
+    def step_each(self, nsteps: int):
+        self._call("lbm_batch_step_each", self._b, int(nsteps))
+
+    def autotune(self, trial_steps: int = 4):
+        t, b = ctypes.c_int(), ctypes.c_int()
+        self._call("lbm_batch_autotune", self._b, int(trial_steps), ctypes.byref(t), ctypes.byref(b))
+        return t.value, b.value
+
+    def set_launch(self, threads: int, blocks: int):
+        self._call("lbm_batch_set_launch", self._b, int(threads), int(blocks))
+
+    def synchronize(self):
+        self._call("lbm_batch_synchronize", self._b)
+
+    def macro(self, m: int):
+        c = self.cfgs[m]
+        out = [np.empty((c.nz, c.ny, c.nx)) for _ in range(4)]
+        self._call("lbm_batch_download_macro", self._b, int(m), *[_dp(a) for a in out])
+        return tuple(out)
+
+    def info(self) -> LbmBatchInfo:
+        inf = LbmBatchInfo()
+        self._call("lbm_batch_get_info", self._b, ctypes.byref(inf))
+        return inf
+
+    def close(self):
+        if self._b:
+            self.lib.lbm_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
 
     def __enter__(self):
         return self
