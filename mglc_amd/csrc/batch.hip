@@ -162,9 +162,10 @@ __global__ __launch_bounds__(256) void k_batch_init(const BatchArgs b) {
         collide<S, COLL>(f, q, a);
         T* dst = static_cast<T*>(b.dst) + in.off;
         const int c = i + in.px * (j + in.ny * k);
+        const unsigned h = cell_random<S>(c, 0u);
         sfor<Q>([&](auto A) {
             constexpr int al = decltype(A)::value;
-            S::st(dst + (long long)al * in.ps + c, f[al], b.scale, store_random<S>(c, al, 0u));
+            S::st(dst + (long long)al * in.ps + c, f[al], b.scale, store_random<S, al>(h));
         });
     }
 }

@@ -147,9 +147,10 @@ __global__ __launch_bounds__(256) void k_cavity_init(const StepArgs a, const dou
     if (k == a.nz - 1) a.lid_rho[i + a.nx * j] = rho;
     collide<S, COLL>(f, q, a);
     T* __restrict__ dst = static_cast<T*>(a.dst);
+    const unsigned h = cell_random<S>(c, a.seed);
     sfor<Q>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        S::st(dst + (long long)al * a.ps + c, f[al], a.scale, store_random<S>(c, al, a.seed));
+        S::st(dst + (long long)al * a.ps + c, f[al], a.scale, store_random<S, al>(h));
     });
 }
 

@@ -106,11 +106,22 @@ struct StoreF16 {
     }
 };
 
-// Random word for the store of population a of cell c at step `seed`.
+// Random words for the stores of cell c at step `seed`: one avalanche hash
+// per cell, then a Weyl step per population (the conversion uses the top 13
+// bits).  Each word is uniform over cells and steps, which is what makes the
+// rounding unbiased; one hash per cell keeps the store ALU-cheap.
 template <class S>
-__device__ __forceinline__ unsigned store_random(long long c, int a, unsigned seed) {
+__device__ __forceinline__ unsigned cell_random(long long c, unsigned seed) {
     if constexpr (S::RANDOM)
-        return mix32((unsigned)c * 19u + (unsigned)a + seed * 0x9E3779B9u);
+        return mix32((unsigned)c ^ (seed * 0x9E3779B9u));
+    else
+        return 0u;
+}
+
+template <class S, int a>
+__device__ __forceinline__ unsigned store_random(unsigned h) {
+    if constexpr (S::RANDOM)
+        return (h + (unsigned)a * 0x61C88647u) >> 19;
     else
         return 0u;
 }

@@ -151,9 +151,10 @@ __device__ __forceinline__ void cavity_cell(const Args& a, int i, int j, int k) 
         a.w[m] = double(q.w);
     }
     collide<S, COLL>(f, q, a);
+    const unsigned h = cell_random<S>(c, a.seed);
     sfor<Q>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        S::st(dst + (long long)al * a.ps + c, f[al], a.scale, store_random<S>(c, al, a.seed));
+        S::st(dst + (long long)al * a.ps + c, f[al], a.scale, store_random<S, al>(h));
     });
 }
 

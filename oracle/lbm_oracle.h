@@ -96,6 +96,32 @@ void oracle_d3q7_basis(double N[7][7], double Ninv[7][7]);
 void oracle_thermal_nusselt(const oracle_thermal_params *p, const double *ua, const double *T,
                             double kappa, double *nu_vol, double *nu_wall);
 
+/* ---- moving spheres (D3Q19 generalisation of case4/mpi) ------------------- */
+typedef struct {
+    int nx, ny, nz;
+    int np;              /* number of spheres */
+    int collision;       /* ORACLE_MRT or ORACLE_BGK */
+    int meq12_rho;
+    double tau;
+    double rho0, rho_solid;
+    double gravity;      /* along -z */
+    double threshold_wall, stiff_wall, threshold_particle, stiff_particle;
+} oracle_particle_params;
+
+/* owner[c] = index of the sphere containing node c (1-based coordinates
+ * i+1, j+1, k+1), -1 for fluid. */
+void oracle_particle_map(const oracle_particle_params *p, const double *xc, const double *r, int *owner);
+int oracle_calq(const double *xc, const double *r, int id, const double x[3], int a, double x0[3],
+                double *q);
+double oracle_calq_closed(const double *xc, const double *r, int id, const double x[3], int a);
+/* nsteps steps from rest; xc, U, Om (3 per sphere) are updated in place;
+ * F, T receive the last step's total force and the hydrodynamic torque.
+ * Returns 0, -1 (allocation), -2 (interpenetration), -3 (wall penetration),
+ * -4 (wall distance q outside [0, 1]). */
+int oracle_particle_run(const oracle_particle_params *p, int nsteps, double *xc, double *U, double *Om,
+                        const double *r, double *F, double *T, int *owner, double *rho, double *u,
+                        double *v, double *w, double *rho_avg);
+
 /* Wall-clock seconds spent inside the step loop of the last run. */
 double oracle_last_loop_seconds(void);
 

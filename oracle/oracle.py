@@ -218,3 +218,79 @@ def run_reference_thermal(nx, ny, nz, nsteps, outfile):
     keys = ("tauf", "paraA", "gBeta", "omegaRatating", "Thot", "Tcold", "Tref", "diffusivity")
     fields = [body[i * n:(i + 1) * n].reshape(nz, ny, nx) for i in range(5)]
     return dict(zip(keys, head)), fields
+
+
+# ---- moving spheres ------------------------------------------------------------
+class ParticleParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("np", ctypes.c_int),
+        ("collision", ctypes.c_int), ("meq12_rho", ctypes.c_int),
+        ("tau", ctypes.c_double), ("rho0", ctypes.c_double), ("rho_solid", ctypes.c_double),
+        ("gravity", ctypes.c_double), ("threshold_wall", ctypes.c_double), ("stiff_wall", ctypes.c_double),
+        ("threshold_particle", ctypes.c_double), ("stiff_particle", ctypes.c_double),
+    ]
+
+
+def particle_params(nx, ny, nz, np_, tau, rho_solid=1.5, gravity=1e-5, rho0=1.0, collision=MRT, meq12_rho=1,
+                    threshold_wall=4.0, stiff_wall=0.10, threshold_particle=4.0, stiff_particle=0.10):
+    return ParticleParams(nx, ny, nz, np_, collision, meq12_rho, tau, rho0, rho_solid, gravity, threshold_wall,
+                          stiff_wall, threshold_particle, stiff_particle)
+
+
+def _particle_lib():
+    L = lib()
+    if not getattr(L, "_particle_ready", False):
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_particle_map.argtypes = [ctypes.POINTER(ParticleParams), dp, dp, ip]
+        L.oracle_calq.argtypes = [dp, dp, ctypes.c_int, dp, ctypes.c_int, dp, dp]
+        L.oracle_calq.restype = ctypes.c_int
+        L.oracle_calq_closed.argtypes = [dp, dp, ctypes.c_int, dp, ctypes.c_int]
+        L.oracle_calq_closed.restype = ctypes.c_double
+        L.oracle_particle_run.argtypes = [ctypes.POINTER(ParticleParams), ctypes.c_int, dp, dp, dp, dp, dp, dp, ip,
+                                          dp, dp, dp, dp, dp]
+        L.oracle_particle_run.restype = ctypes.c_int
+        L._particle_ready = True
+    return L
+
+
+def _ip(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def particle_map(p: ParticleParams, xc, r):
+    owner = np.zeros((p.nz, p.ny, p.nx), dtype=np.int32)
+    xc = np.ascontiguousarray(xc, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    _particle_lib().oracle_particle_map(ctypes.byref(p), _dp(xc), _dp(r), _ip(owner))
+    return owner
+
+
+def calq(xc, r, pid, x, a):
+    xc = np.ascontiguousarray(xc, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    x0 = np.zeros(3)
+    q = ctypes.c_double()
+    rc = _particle_lib().oracle_calq(_dp(xc), _dp(r), pid, _dp(x), a, _dp(x0), ctypes.byref(q))
+    closed = _particle_lib().oracle_calq_closed(_dp(xc), _dp(r), pid, _dp(x), a)
+    return rc, q.value, x0, closed
+
+
+def particle_run(p: ParticleParams, nsteps: int, xc, r, U=None, Om=None):
+    """Spheres at xc (np x 3, 1-based node coordinates) with radii r, fluid at
+    rest; returns a dict of the state after nsteps (rc != 0 on a stop)."""
+    n = p.np
+    xc = np.ascontiguousarray(xc, dtype=np.float64).reshape(n, 3).copy()
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    U = np.zeros((n, 3)) if U is None else np.ascontiguousarray(U, dtype=np.float64).copy()
+    Om = np.zeros((n, 3)) if Om is None else np.ascontiguousarray(Om, dtype=np.float64).copy()
+    F = np.zeros((n, 3)); T = np.zeros((n, 3))
+    shape = (p.nz, p.ny, p.nx)
+    owner = np.zeros(shape, dtype=np.int32)
+    rho, u, v, w = (np.zeros(shape) for _ in range(4))
+    ra = ctypes.c_double()
+    rc = _particle_lib().oracle_particle_run(ctypes.byref(p), nsteps, _dp(xc), _dp(U), _dp(Om), _dp(r), _dp(F), _dp(T),
+                                             _ip(owner), _dp(rho), _dp(u), _dp(v), _dp(w), ctypes.byref(ra))
+    return dict(rc=rc, xc=xc, U=U, Om=Om, F=F, T=T, owner=owner, rho=rho, u=u, v=v, w=w, rho_avg=ra.value)
