@@ -37,7 +37,10 @@ enum lbm_status {
     LBM_ERR_TAU = 4,      /* relaxation time out of range (tau <= 0.5) */
     LBM_ERR_NOMEM = 5,    /* device allocation failed */
     LBM_ERR_STATE = 6,    /* call out of order (e.g. step before initial) */
-    LBM_ERR_UNSUPPORTED = 7
+    LBM_ERR_UNSUPPORTED = 7,
+    LBM_ERR_PARTICLE = 8  /* particle run stopped: interpenetration, wall penetration
+                             or a wall distance q outside [0, 1] (particle_force.F90:117-186,
+                             particle_bounceback.F90:128-138) */
 };
 
 enum lbm_collision { LBM_MRT = 0, LBM_BGK = 1 };
@@ -189,6 +192,43 @@ int lbm_download_macro(lbm_ctx *ctx, double *rho, double *u, double *v, double *
 int lbm_synchronize(lbm_ctx *ctx);
 
 int lbm_get_info(const lbm_ctx *ctx, lbm_info *info);
+
+/* ---- moving spheres --------------------------------------------------------
+ * Rigid spheres in a closed box (no-slip walls, no lid), coupled to the
+ * D3Q19 fluid as the reference couples discs to D2Q9
+ * (MPI/Micro_particles/fortran/case4/mpi: fluid.F90, particle_bounceback.F90,
+ * particle_force.F90, particle_update.F90): collision on fluid nodes, masked
+ * streaming, quadratic interpolated bounce-back with the moving-wall term
+ * (wall distance q by bisection), momentum-exchange force and torque summed
+ * per sphere in FP64 (deterministic block sums), spring and buoyancy forces,
+ * kinematics, new solid map, refill of uncovered nodes.  Coordinates are
+ * 1-based node coordinates (node (i,j,k) sits at (i+1, j+1, k+1)); gravity
+ * acts along -z.  Storage FP64 or FP32. */
+typedef struct lbm_particle_params {
+    int np;
+    int reserved_i;
+    double rho_solid;        /* rhoSolid */
+    double gravity;          /* |g| along -z, lattice units */
+    double threshold_wall, stiff_wall;          /* wall spring (particle_force.F90:126-186) */
+    double threshold_particle, stiff_particle;  /* particle spring (particle_force.F90:96-124) */
+    double reserved_d[4];
+} lbm_particle_params;
+
+/* Attach np spheres (centres np*3, radii np; velocities U and angular
+ * velocities omega np*3 each, NULL for rest) to a context and set the initial
+ * state of initial.F90:78-135: solid map, rho = rho_s in solids and rho0 in
+ * the fluid, u = 0, f = w rho0 on fluid nodes.  Replaces lbm_initial /
+ * lbm_load_fields (which detach the spheres again).  Needs an isothermal
+ * FP64/FP32 context with u_lid = 0 and no halo faces. */
+int lbm_particles_init(lbm_ctx *ctx, const lbm_particle_params *pp, const double *centers,
+                       const double *radii, const double *U, const double *omega);
+/* Sphere state of the last step (each np*3; any pointer may be NULL):
+ * centre, velocity, angular velocity, total force (hydrodynamic + springs +
+ * buoyancy) and hydrodynamic torque. */
+int lbm_particles_get(lbm_ctx *ctx, double *centers, double *U, double *omega, double *F, double *T);
+/* Owner map of the current step: sphere index per node, -1 for fluid
+ * (nx*ny*nz int32, x fastest). */
+int lbm_particles_map(lbm_ctx *ctx, int *owner);
 
 /* ---- batched cavities ----------------------------------------------------
  * Many independent lid-driven cavities of uneven sizes, Re and U0 stepped

@@ -373,6 +373,7 @@ void release(lbm_ctx* c) {
     if (c->hin) (void)hipFree(c->hin);
     if (c->hsend) (void)hipFree(c->hsend);
     thermal_release(c);
+    particles_release(c);
 }
 
 }  // namespace
@@ -586,6 +587,7 @@ static int load_device_fields(lbm_ctx* c, const double* d_rho, const double* d_u
 
 int lbm_initial(lbm_ctx* c) {
     if (!c) return lbm_fail(LBM_ERR_ARG, "lbm_initial: null context");
+    particles_release(c);  // a fresh start detaches any spheres
     // initial fields into the macro arrays, then equilibrium + first collision
     // from them.  The init kernel reads and rewrites the same macro values
     // cell by cell, so the arrays can serve as both input and output.
@@ -612,6 +614,7 @@ int lbm_initial(lbm_ctx* c) {
 int lbm_load_fields(lbm_ctx* c, const double* rho, const double* u, const double* v, const double* w,
                     const double* T) {
     if (!c || !rho || !u || !v || !w) return lbm_fail(LBM_ERR_ARG, "lbm_load_fields: null argument");
+    particles_release(c);
     const size_t bytes = sizeof(double) * c->n;
     if (c->p.thermal) {
         if (T)
@@ -635,6 +638,11 @@ int lbm_step(lbm_ctx* c, int nsteps) {
     if (!c->initialised) return lbm_fail(LBM_ERR_STATE, "lbm_step before lbm_initial / lbm_load_fields");
     if (c->p.thermal) {
         int rc = thermal_step(c, nsteps);
+        c->itc += nsteps;
+        return rc;
+    }
+    if (c->part) {
+        int rc = particles_step(c, nsteps);
         c->itc += nsteps;
         return rc;
     }

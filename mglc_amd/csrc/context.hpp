@@ -25,6 +25,8 @@ constexpr int RED_THREADS = 256;
 constexpr int RED_BLOCKS = 1024;
 constexpr int RED_MAX_SUMS = 4;
 
+struct ParticleState;  // moving spheres (particles.hip)
+
 struct lbm_ctx {
     lbm_params p{};
     int device = 0;
@@ -53,6 +55,7 @@ struct lbm_ctx {
     double* force = nullptr;     // Fx, Fy, Fz of the last collision (3 * ps), in place
     double* temp = nullptr;      // T (n)
     double* prev_t = nullptr;    // wp, Tp for the thermal check (2 * n)
+    ParticleState* part = nullptr;  // moving spheres, when attached
 };
 
 // Thermal path (thermal.hip).
@@ -63,4 +66,8 @@ int thermal_load(lbm_ctx* c, const double* d_rho, const double* d_u, const doubl
 int thermal_step(lbm_ctx* c, int nsteps);
 int thermal_check(lbm_ctx* c, double* sums4);
 int thermal_nusselt(lbm_ctx* c, double* nu_vol, double* nu_wall);
+
+// Particle path (particles.hip).
+int particles_step(lbm_ctx* c, int nsteps);
+void particles_release(lbm_ctx* c);
 

@@ -1,0 +1,836 @@
+// particles.hip -- rigid spheres coupled to the D3Q19 fluid on one MI355X, as
+// the reference couples discs to D2Q9 (MPI/Micro_particles/fortran/case4/mpi).
+//
+// Per step, all on the device (no host round trip):
+//   k_fluid_mean      rhoAvg over fluid nodes of the last macro
+//                     (particle_bounceback.F90:15-35)
+//   k_particle_links  one block per sphere: its boundary links compacted in
+//                     LDS; per link the wall distance q by bisection, the
+//                     quadratic interpolated bounce-back value with the
+//                     moving-wall term (particle_bounceback.F90:37-141), and
+//                     the momentum exchange force and torque
+//                     (particle_force.F90:31-92) summed in FP64 in a fixed
+//                     order (deterministic)
+//   k_particle_step   per fluid node: masked pull streaming with box-wall
+//                     bounce-back and the link values, macro, collision
+//   k_particle_update one thread per sphere: spring and buoyancy forces and
+//                     the kinematics (particle_force.F90:96-191,
+//                     particle_update.F90:26-42); stop codes for
+//                     interpenetration and wall penetration
+//   k_particle_mark   new solid map, rho_s and u = 0 in solids
+//                     (particle_update.F90:69-91)
+//   k_fluid_mean      rhoAvg over the new fluid nodes
+//   k_particle_refill one block per sphere: uncovered nodes refilled by 3-point
+//                     extrapolation along the most outward lattice direction
+//                     and moment replacement with the body velocity
+//                     (particle_update.F90:110-186), then collided
+// The 3-D choices (volume 4/3 pi r^3, I = 2/5 m r^2, springs on all six
+// walls, rest equilibrium for stencil points that are not fluid nodes) are
+// those of the CPU restatement the GPU path is tested against.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <new>
+
+#include "collide.hpp"
+#include "context.hpp"
+#include "step.hpp"
+
+using namespace lbm;
+
+struct ParticleState {
+    int np = 0;
+    lbm_particle_params pp{};
+    double *xc[2] = {nullptr, nullptr}, *U[2] = {nullptr, nullptr}, *Om[2] = {nullptr, nullptr};
+    double* r = nullptr;
+    double* F = nullptr;     // hydrodynamic force (3 np) of the last step
+    double* T = nullptr;     // hydrodynamic torque (3 np)
+    double* Ftot = nullptr;  // total force (3 np)
+    int* owner[2] = {nullptr, nullptr};
+    int cur = 0;             // current map / sphere state
+    void* fstream = nullptr; // link values: f(abar, x) on links out of a solid (storage type, 19 ps)
+    double* scal = nullptr;  // [0] rhoAvg of the streaming, [1] rhoAvg of the refill
+    double* red = nullptr;   // reduction partials
+    int* err = nullptr;      // device stop code (0 = running)
+};
+
+namespace {
+
+constexpr double PI = 3.14159265358979323846;
+
+struct PArgs {
+    const void* src;
+    void* dst;
+    void* fstream;
+    int nx, ny, nz, px;
+    long long pxy, ps;
+    double snu, sq;
+    float scale, inv_scale;
+    int meq12_rho;
+    double rho0, rho_solid;
+    const int* owner;      // map of this step
+    const int* owner_new;  // map after the sphere update (refill)
+    const double *xc, *U, *Om;           // sphere state of this step
+    const double *xc_new, *U_new, *Om_new;  // after the update (refill)
+    const double* r;
+    const double* rho_avg;  // device scalars: [0] streaming, [1] refill
+    double *rho, *u, *v, *w;
+    int* err;
+    unsigned seed;
+    int np;
+    // sphere update
+    double gravity, threshold_wall, stiff_wall, threshold_particle, stiff_particle;
+    double *F, *T, *Ftot;
+    double *xc_out, *U_out, *Om_out;
+};
+
+__device__ __forceinline__ double dist2(double dx, double dy, double dz) {
+    // products rounded separately and summed left to right, as the reference's
+    // x**2.0d0 sums compile at -O3: the solid map is bit-exact
+    return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+}
+
+// calQ (particle_bounceback.F90:101-141) for the link from node X along
+// e = (ex, ey, ez) into sphere (cx, cy, cz, rad).  Returns false when q
+// leaves [0, 1].
+__device__ __noinline__ bool calq(double cx, double cy, double cz, double rad, double X, double Y, double Z, int ex, int ey,
+                     int ez, double& x0, double& y0, double& z0, double& qout) {
+    double q = 0.5, qt = 0.5;
+    double px = X + qt * ex, py = Y + qt * ey, pz = Z + qt * ez;
+    for (int it = 0; it < 200; ++it) {
+        const double d = sqrt(dist2(px - cx, py - cy, pz - cz));
+        if (fabs(d - rad) < 1e-9) break;
+        qt = qt / 2.0;
+        if (d > rad) {
+            px += qt * ex;
+            py += qt * ey;
+            pz += qt * ez;
+            q += qt;
+        } else {
+            px -= qt * ex;
+            py -= qt * ey;
+            pz -= qt * ez;
+            q -= qt;
+        }
+    }
+    x0 = px;
+    y0 = py;
+    z0 = pz;
+    qout = q;
+    return !(q > 1.0 || q < 0.0);
+}
+
+__device__ __forceinline__ void body_velocity(const double* xc, const double* U, const double* Om, int id, double x,
+                                              double y, double z, double& ux, double& uy, double& uz) {
+    const double rx = x - xc[3 * id], ry = y - xc[3 * id + 1], rz = z - xc[3 * id + 2];
+    ux = U[3 * id] + Om[3 * id + 1] * rz - Om[3 * id + 2] * ry;
+    uy = U[3 * id + 1] + Om[3 * id + 2] * rx - Om[3 * id] * rz;
+    uz = U[3 * id + 2] + Om[3 * id] * ry - Om[3 * id + 1] * rx;
+}
+
+__device__ __forceinline__ bool in_box(const PArgs& a, int i, int j, int k) {
+    return (unsigned)i < (unsigned)a.nx && (unsigned)j < (unsigned)a.ny && (unsigned)k < (unsigned)a.nz;
+}
+
+// Absolute population a of a stored state at storage index c.
+template <class S>
+__device__ __forceinline__ double abs_pop(const typename S::T* buf, long long ps, long long c, int a, float inv) {
+    const double g = double(S::ld(buf + (long long)a * ps + c, inv));
+    return S::SHIFT ? g + weight(a) : g;
+}
+
+// f_post of node (i,j,k) for an interpolation stencil: the rest equilibrium
+// w_a rho0 where the point is not a fluid node.
+template <class S>
+__device__ __forceinline__ double stencil_fpost(const PArgs& a, const typename S::T* src, int i, int j, int k, int al) {
+    if (!in_box(a, i, j, k)) return weight(al) * a.rho0;
+    const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
+    if (a.owner[m] >= 0) return weight(al) * a.rho0;
+    const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
+    return abs_pop<S>(src, a.ps, c, al, a.inv_scale);
+}
+
+// Bounding box (0-based cell indices, clamped) of sphere `id` with `margin`.
+__device__ __forceinline__ void sphere_box(const PArgs& a, const double* xc, int id, int margin, int lo[3], int hi[3]) {
+    const int n[3] = {a.nx, a.ny, a.nz};
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = (int)floor(xc[3 * id + d] - a.r[id]) - 1 - margin;
+        hi[d] = (int)ceil(xc[3 * id + d] + a.r[id]) - 1 + margin;
+        lo[d] = lo[d] < 0 ? 0 : lo[d];
+        hi[d] = hi[d] > n[d] - 1 ? n[d] - 1 : hi[d];
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    return x;
+}
+
+constexpr int LINK_THREADS = 256;
+constexpr int LINK_CAP = 8192;                     // LDS link list (32 KiB)
+constexpr int LINK_CHUNK_MAX = LINK_THREADS * 18;  // links one chunk of nodes can add
+
+// Boundary links of one sphere: every (fluid node x, direction a) with x + e_a
+// owned by the sphere.  The block walks the sphere's bounding box in chunks of
+// 256 nodes, compacts the links into an LDS list (block prefix sum), and then
+// processes the list densely, so the bisection for q runs on full waves:
+//   - wall distance q and boundary point x0 (calQ, particle_bounceback.F90:101-141)
+//   - the streamed population f(abar, x) by quadratic interpolated bounce-back
+//     with the moving-wall term (particle_bounceback.F90:37-96), written to
+//     the streamed-population buffer for the fluid kernel to pull
+//   - the link's momentum exchange (e_a - u_w) f_post(a) - (e_abar - u_w) f(abar)
+//     and its torque (particle_force.F90:31-92), summed in FP64 per thread in
+//     list order, then over the block in a fixed tree: deterministic.
+template <class S>
+__global__ __launch_bounds__(LINK_THREADS) void k_particle_links(const PArgs a) {
+    using T = typename S::T;
+    using R = typename S::Real;
+    const int id = blockIdx.x;
+    int lo[3], hi[3];
+    sphere_box(a, a.xc, id, 1, lo, hi);
+    const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
+    const long long total = (bx > 0 && by > 0 && bz > 0) ? (long long)bx * by * bz : 0;
+    const T* src = static_cast<const T*>(a.src);
+    T* fs = static_cast<T*>(a.fstream);
+    const double cx = a.xc[3 * id], cy = a.xc[3 * id + 1], cz = a.xc[3 * id + 2], rad = a.r[id];
+    const double rho_avg = a.rho_avg[0];
+    __shared__ unsigned items[LINK_CAP];
+    __shared__ int wave_tot[LINK_THREADS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    int count = 0;  // list length, the same in every thread
+    const long long nchunks = (total + LINK_THREADS - 1) / LINK_THREADS;
+    for (long long ch = 0; ch < nchunks; ++ch) {
+        const long long t = ch * LINK_THREADS + threadIdx.x;
+        unsigned mask = 0;
+        if (t < total) {
+            const int i = lo[0] + (int)(t % bx), j = lo[1] + (int)((t / bx) % by),
+                      k = lo[2] + (int)(t / ((long long)bx * by));
+            if (a.owner[i + (long long)a.nx * (j + (long long)a.ny * k)] < 0) {
+                sfor<Q - 1>([&](auto B) {
+                    constexpr int al = decltype(B)::value + 1;
+                    const int ti = i + EX[al], tj = j + EY[al], tk = k + EZ[al];
+                    if (in_box(a, ti, tj, tk) && a.owner[ti + (long long)a.nx * (tj + (long long)a.ny * tk)] == id)
+                        mask |= 1u << al;
+                });
+            }
+        }
+        const int cnt = __popc(mask);
+        int incl = cnt;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += y;
+        }
+        if (lane == 63) wave_tot[wid] = incl;
+        __syncthreads();
+        int pos = count + incl - cnt, chunk_total = 0;
+        for (int w = 0; w < LINK_THREADS / 64; ++w) {
+            if (w < wid) pos += wave_tot[w];
+            chunk_total += wave_tot[w];
+        }
+        while (mask) {
+            const int al = __ffs(mask) - 1;
+            mask &= mask - 1;
+            items[pos++] = ((unsigned)t << 5) | (unsigned)al;
+        }
+        count += chunk_total;
+        __syncthreads();
+        if (count <= LINK_CAP - LINK_CHUNK_MAX && ch != nchunks - 1) continue;
+        for (int q = threadIdx.x; q < count; q += LINK_THREADS) {
+            const unsigned it = items[q];
+            const long long tt = it >> 5;
+            const int al = (int)(it & 31u), bd = OPP[al];
+            const int i = lo[0] + (int)(tt % bx), j = lo[1] + (int)((tt / bx) % by),
+                      k = lo[2] + (int)(tt / ((long long)bx * by));
+            const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
+            double x0, y0, z0, qq;
+            if (!calq(cx, cy, cz, rad, i + 1.0, j + 1.0, k + 1.0, EX[al], EY[al], EZ[al], x0, y0, z0, qq))
+                atomicMax(a.err, 3);
+            double uwx, uwy, uwz;
+            body_velocity(a.xc, a.U, a.Om, id, x0, y0, z0, uwx, uwy, uwz);
+            const double eu = EX[bd] * uwx + EY[bd] * uwy + EZ[bd] * uwz;
+            const double fa = abs_pop<S>(src, a.ps, c, al, a.inv_scale);
+            double val;
+            if (qq < 0.5) {
+                const double f1 = stencil_fpost<S>(a, src, i - EX[al], j - EY[al], k - EZ[al], al);
+                const double f2 = stencil_fpost<S>(a, src, i - 2 * EX[al], j - 2 * EY[al], k - 2 * EZ[al], al);
+                val = qq * (1.0 + 2.0 * qq) * fa + (1.0 - 4.0 * qq * qq) * f1 - qq * (1.0 - 2.0 * qq) * f2 +
+                      6.0 * weight(al) * rho_avg * eu;
+            } else {
+                const double fb = abs_pop<S>(src, a.ps, c, bd, a.inv_scale);
+                const double fb1 = stencil_fpost<S>(a, src, i - EX[al], j - EY[al], k - EZ[al], bd);
+                val = fa / qq / (1.0 + 2.0 * qq) + fb * (2.0 * qq - 1.0) / qq -
+                      fb1 * (2.0 * qq - 1.0) / (2.0 * qq + 1.0) + 6.0 * weight(al) * rho_avg / qq / (1.0 + 2.0 * qq) * eu;
+            }
+            S::st(fs + (long long)bd * a.ps + c, S::SHIFT ? R(val - weight(bd)) : R(val), a.scale, 0u);
+            const double dFx = (EX[al] - uwx) * fa - (EX[bd] - uwx) * val;
+            const double dFy = (EY[al] - uwy) * fa - (EY[bd] - uwy) * val;
+            const double dFz = (EZ[al] - uwz) * fa - (EZ[bd] - uwz) * val;
+            const double rx = x0 - cx, ry = y0 - cy, rz = z0 - cz;
+            acc[0] += dFx;
+            acc[1] += dFy;
+            acc[2] += dFz;
+            acc[3] += ry * dFz - rz * dFy;
+            acc[4] += rz * dFx - rx * dFz;
+            acc[5] += rx * dFy - ry * dFx;
+        }
+        count = 0;
+        __syncthreads();  // the next chunk refills the list
+    }
+    __shared__ double sh[6][LINK_THREADS / 64];
+    for (int q = 0; q < 6; ++q) {
+        const double v = wave_sum(acc[q]);
+        if (lane == 0) sh[q][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 6; ++q) {
+            double v = 0.0;
+            for (int w = 0; w < LINK_THREADS / 64; ++w) v += sh[q][w];
+            if (q < 3) a.F[3 * id + q] = v;
+            else a.T[3 * id + q - 3] = v;
+        }
+    }
+}
+
+// Fluid nodes: masked pull streaming (fluid.F90:75-161) with halfway bounce-
+// back at the box walls and, on links whose source is a solid node, the
+// interpolated bounce-back value left by k_particle_links; macro
+// (fluid.F90:164-184) and the next collision (fluid.F90:1-72).  Solid nodes
+// are skipped.
+template <class S, int COLL>
+__global__ __launch_bounds__(256) void k_particle_step(const PArgs a) {
+    using T = typename S::T;
+    using R = typename S::Real;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= a.nx || j >= a.ny) return;
+    const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
+    if (a.owner[m] >= 0) return;
+    const int c = i + a.px * (j + a.ny * k);
+    const T* __restrict__ src = static_cast<const T*>(a.src);
+    const T* __restrict__ fs = static_cast<const T*>(a.fstream);
+    R f[Q];
+    sfor<Q>([&](auto B) {
+        constexpr int bd = decltype(B)::value;
+        constexpr int ex = EX[bd], ey = EY[bd], ez = EZ[bd];
+        if constexpr (bd == 0) {
+            f[0] = S::ld(src + c, a.inv_scale);
+        } else {
+            const int si = i - ex, sj = j - ey, sk = k - ez;
+            if (!in_box(a, si, sj, sk))
+                f[bd] = S::ld(src + (long long)OPP[bd] * a.ps + c, a.inv_scale);
+            else if (a.owner[si + (long long)a.nx * (sj + (long long)a.ny * sk)] >= 0)
+                f[bd] = S::ld(fs + (long long)bd * a.ps + c, a.inv_scale);
+            else
+                f[bd] = S::ld(src + (long long)bd * a.ps +
+                                  (c - ((long long)ex + (long long)ey * a.px + (long long)ez * a.pxy)),
+                              a.inv_scale);
+        }
+    });
+    const Moments<R, S::SHIFT> mo = moments<R, S::SHIFT>(f);
+    a.rho[m] = S::SHIFT ? 1.0 + double(mo.drho) : double(mo.rho);
+    a.u[m] = double(mo.u);
+    a.v[m] = double(mo.v);
+    a.w[m] = double(mo.w);
+    collide<S, COLL>(f, mo, a);
+    T* __restrict__ dst = static_cast<T*>(a.dst);
+    sfor<Q>([&](auto B) {
+        constexpr int bd = decltype(B)::value;
+        S::st(dst + (long long)bd * a.ps + c, f[bd], a.scale, 0u);
+    });
+}
+
+// Streamed population bd of fluid node (i,j,k), as k_particle_step pulls it.
+template <class S>
+__device__ double streamed_pop(const PArgs& a, const typename S::T* src, const typename S::T* fs, int i, int j, int k,
+                               int bd) {
+    const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
+    if (bd == 0) return abs_pop<S>(src, a.ps, c, 0, a.inv_scale);
+    const int si = i - EX[bd], sj = j - EY[bd], sk = k - EZ[bd];
+    if (!in_box(a, si, sj, sk)) return abs_pop<S>(src, a.ps, c, OPP[bd], a.inv_scale);
+    if (a.owner[si + (long long)a.nx * (sj + (long long)a.ny * sk)] >= 0) return abs_pop<S>(fs, a.ps, c, bd, a.inv_scale);
+    return abs_pop<S>(src, a.ps, si + (long long)a.px * (sj + (long long)a.ny * sk), bd, a.inv_scale);
+}
+
+__global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.np) return;
+    const double rho_avg = a.rho_avg[0];
+    const double rp = a.r[p];
+    const double V = 4.0 / 3.0 * PI * rp * rp * rp;
+    double fs[3] = {0.0, 0.0, 0.0};
+    const double scale_p = V * (a.rho_solid - rho_avg) * a.gravity / a.stiff_particle;
+    for (int q = 0; q < a.np; ++q) {
+        if (q == p) continue;
+        const double dx = a.xc[3 * p] - a.xc[3 * q], dy = a.xc[3 * p + 1] - a.xc[3 * q + 1],
+                     dz = a.xc[3 * p + 2] - a.xc[3 * q + 2];
+        const double dij = sqrt(dist2(dx, dy, dz));
+        const double rs = rp + a.r[q];
+        if (dij >= rs + a.threshold_particle) continue;
+        if (dij < rs) {
+            atomicMax(a.err, 1);
+            continue;
+        }
+        const double g = (dij - rs - a.threshold_particle) / a.threshold_particle;
+        const double mag = scale_p * (g * g);
+        fs[0] += mag * dx / dij;
+        fs[1] += mag * dy / dij;
+        fs[2] += mag * dz / dij;
+    }
+    const double scale_w = V * (a.rho_solid - a.rho0) * a.gravity / a.stiff_wall;
+    const double N[3] = {(double)a.nx, (double)a.ny, (double)a.nz};
+    for (int d = 0; d < 3; ++d) {
+        const double lo = a.xc[3 * p + d] - rp - 1.0;
+        const double hi = N[d] - a.xc[3 * p + d] - rp;
+        if (lo < 0.0 || hi < 0.0) atomicMax(a.err, 2);
+        if (lo < a.threshold_wall) {
+            const double g = (lo - a.threshold_wall) / a.threshold_wall;
+            fs[d] += scale_w * (g * g);
+        }
+        if (hi < a.threshold_wall) {
+            const double g = (hi - a.threshold_wall) / a.threshold_wall;
+            fs[d] -= scale_w * (g * g);
+        }
+    }
+    double Fn[3] = {a.F[3 * p] + fs[0], a.F[3 * p + 1] + fs[1],
+                    a.F[3 * p + 2] - (a.rho_solid - rho_avg) * V * a.gravity + fs[2]};
+    const double mass = a.rho_solid * V, inertia = 0.4 * mass * rp * rp;
+    for (int d = 0; d < 3; ++d) {
+        const double acc = Fn[d] / mass;
+        const double Uold = a.U[3 * p + d];
+        a.U_out[3 * p + d] = Uold + acc;
+        a.xc_out[3 * p + d] = a.xc[3 * p + d] + Uold + 0.5 * acc;
+        a.Om_out[3 * p + d] = a.Om[3 * p + d] + a.T[3 * p + d] / inertia;
+        a.Ftot[3 * p + d] = Fn[d];
+    }
+}
+
+// New map from the updated spheres; solids at rest with rho_s.
+__global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner_out) {
+    const int id = blockIdx.x;
+    int lo[3], hi[3];
+    sphere_box(a, a.xc_new, id, 1, lo, hi);
+    const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
+    const long long total = (bx > 0 && by > 0 && bz > 0) ? (long long)bx * by * bz : 0;
+    const double cx = a.xc_new[3 * id], cy = a.xc_new[3 * id + 1], cz = a.xc_new[3 * id + 2], rr = a.r[id] * a.r[id];
+    for (long long t = threadIdx.x; t < total; t += blockDim.x) {
+        const int i = lo[0] + (int)(t % bx), j = lo[1] + (int)((t / bx) % by), k = lo[2] + (int)(t / ((long long)bx * by));
+        if (dist2(i + 1.0 - cx, j + 1.0 - cy, k + 1.0 - cz) <= rr) {
+            const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
+            atomicMax(owner_out + m, id);  // a node claimed twice goes to the higher index
+            a.rho[m] = a.rho_solid;
+            a.u[m] = 0.0;
+            a.v[m] = 0.0;
+            a.w[m] = 0.0;
+        }
+    }
+}
+
+// Initial map (from the current state) and the fluid equilibrium at rest.
+__global__ __launch_bounds__(256) void k_particle_init_mark(const PArgs a, int* owner_out) {
+    const int id = blockIdx.x;
+    int lo[3], hi[3];
+    sphere_box(a, a.xc, id, 1, lo, hi);
+    const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
+    const long long total = (bx > 0 && by > 0 && bz > 0) ? (long long)bx * by * bz : 0;
+    const double cx = a.xc[3 * id], cy = a.xc[3 * id + 1], cz = a.xc[3 * id + 2], rr = a.r[id] * a.r[id];
+    for (long long t = threadIdx.x; t < total; t += blockDim.x) {
+        const int i = lo[0] + (int)(t % bx), j = lo[1] + (int)((t / bx) % by), k = lo[2] + (int)(t / ((long long)bx * by));
+        if (dist2(i + 1.0 - cx, j + 1.0 - cy, k + 1.0 - cz) <= rr)
+            atomicMax(owner_out + i + (long long)a.nx * (j + (long long)a.ny * k), id);
+    }
+}
+
+// rho = rho_s in solids and rho0 in the fluid, u = 0, and the first
+// collision of f = w rho0 on fluid nodes written to dst.
+template <class S, int COLL>
+__global__ __launch_bounds__(256) void k_particle_init_fields(const PArgs a) {
+    using T = typename S::T;
+    using R = typename S::Real;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= a.nx || j >= a.ny) return;
+    const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
+    const bool solid = a.owner[m] >= 0;
+    a.rho[m] = solid ? a.rho_solid : a.rho0;
+    a.u[m] = 0.0;
+    a.v[m] = 0.0;
+    a.w[m] = 0.0;
+    if (solid) return;
+    R f[Q];
+    sfor<Q>([&](auto B) {
+        constexpr int bd = decltype(B)::value;
+        const double fe = weight(bd) * a.rho0;
+        f[bd] = S::SHIFT ? R(fe - weight(bd)) : R(fe);
+    });
+    Moments<R, S::SHIFT> mo;
+    mo.rho = R(a.rho0);
+    mo.drho = R(a.rho0 - 1.0);
+    mo.u = mo.v = mo.w = R(0);
+    collide<S, COLL>(f, mo, a);
+    const int c = i + a.px * (j + a.ny * k);
+    T* __restrict__ dst = static_cast<T*>(a.dst);
+    sfor<Q>([&](auto B) {
+        constexpr int bd = decltype(B)::value;
+        S::st(dst + (long long)bd * a.ps + c, f[bd], a.scale, 0u);
+    });
+}
+
+template <class S, int COLL>
+__global__ __launch_bounds__(256) void k_particle_refill(const PArgs a) {
+    using T = typename S::T;
+    using R = typename S::Real;
+    const int id = blockIdx.x;
+    int lo[3], hi[3];
+    sphere_box(a, a.xc, id, 1, lo, hi);  // old position
+    const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
+    const long long total = (bx > 0 && by > 0 && bz > 0) ? (long long)bx * by * bz : 0;
+    const T* src = static_cast<const T*>(a.src);
+    const T* fs = static_cast<const T*>(a.fstream);
+    const double rho_avg = a.rho_avg[1];
+    for (long long t = threadIdx.x; t < total; t += blockDim.x) {
+        const int i = lo[0] + (int)(t % bx), j = lo[1] + (int)((t / bx) % by), k = lo[2] + (int)(t / ((long long)bx * by));
+        const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
+        if (!(a.owner[m] == id && a.owner_new[m] < 0)) continue;
+        // extrapolation direction: most outward lattice direction w.r.t. the new centre
+        const double rx = i + 1.0 - a.xc_new[3 * id], ry = j + 1.0 - a.xc_new[3 * id + 1],
+                     rz = k + 1.0 - a.xc_new[3 * id + 2];
+        const double rn = sqrt(dist2(rx, ry, rz));
+        double best = 0.0;
+        int ec = 0;
+        for (int al = 1; al < Q; ++al) {
+            const double tn = __dadd_rn(__dadd_rn(rx * EX[al], ry * EY[al]), rz * EZ[al]) / rn;
+            if (tn > best) {
+                best = tn;
+                ec = al;
+            }
+        }
+        if (ec == 0) {
+            atomicMax(a.err, 4);
+            continue;
+        }
+        double fr[Q];
+        for (int al = 0; al < Q; ++al) fr[al] = 0.0;
+        for (int s = 1; s <= 3; ++s) {
+            const int pi = i + s * EX[ec], pj = j + s * EY[ec], pk = k + s * EZ[ec];
+            bool ok = in_box(a, pi, pj, pk);
+            if (ok) {
+                const long long pm = pi + (long long)a.nx * (pj + (long long)a.ny * pk);
+                ok = a.owner[pm] < 0 && a.owner_new[pm] < 0;
+            }
+            const double coef = s == 1 ? 3.0 : (s == 2 ? -3.0 : 1.0);
+            for (int al = 0; al < Q; ++al) {
+                const double v = ok ? streamed_pop<S>(a, src, fs, pi, pj, pk, al) : weight(al) * a.rho0;
+                fr[al] = s == 1 ? coef * v : fr[al] + coef * v;
+            }
+        }
+        // momentum moments replaced by rhoAvg times the body velocity
+        double mm[Q];
+        mrt_forward(fr, mm);
+        double ux, uy, uz;
+        body_velocity(a.xc_new, a.U_new, a.Om_new, id, i + 1.0, j + 1.0, k + 1.0, ux, uy, uz);
+        mm[3] = rho_avg * ux;
+        mm[5] = rho_avg * uy;
+        mm[7] = rho_avg * uz;
+        mrt_inverse(mm, fr);
+        R f[Q];
+        for (int al = 0; al < Q; ++al) f[al] = S::SHIFT ? R(fr[al] - weight(al)) : R(fr[al]);
+        const Moments<R, S::SHIFT> mo = moments<R, S::SHIFT>(f);
+        a.rho[m] = S::SHIFT ? 1.0 + double(mo.drho) : double(mo.rho);
+        a.u[m] = double(mo.u);
+        a.v[m] = double(mo.v);
+        a.w[m] = double(mo.w);
+        collide<S, COLL>(f, mo, a);
+        const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
+        T* __restrict__ dst = static_cast<T*>(a.dst);
+        for (int al = 0; al < Q; ++al) S::st(dst + (long long)al * a.ps + c, f[al], a.scale, 0u);
+    }
+}
+
+// Sum of rho and count over fluid nodes, then rhoAvg into out.
+__global__ __launch_bounds__(RED_THREADS) void k_fluid_sum(const double* __restrict__ rho, const int* __restrict__ owner,
+                                                           long long n, double* __restrict__ partial) {
+    double s = 0.0, cnt = 0.0;
+    for (long long c = blockIdx.x * (long long)RED_THREADS + threadIdx.x; c < n; c += (long long)gridDim.x * RED_THREADS)
+        if (owner[c] < 0) {
+            s += rho[c];
+            cnt += 1.0;
+        }
+    __shared__ double sh[2][RED_THREADS / 64];
+    s = wave_sum(s);
+    cnt = wave_sum(cnt);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][wid] = s;
+        sh[1][wid] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < RED_THREADS / 64; ++w) {
+            a += sh[0][w];
+            b += sh[1][w];
+        }
+        partial[2 * blockIdx.x] = a;
+        partial[2 * blockIdx.x + 1] = b;
+    }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_fluid_mean(const double* __restrict__ partial, int nblocks,
+                                                            double* __restrict__ out) {
+    double s = 0.0, cnt = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += RED_THREADS) {
+        s += partial[2 * b];
+        cnt += partial[2 * b + 1];
+    }
+    __shared__ double sh[2][RED_THREADS / 64];
+    s = wave_sum(s);
+    cnt = wave_sum(cnt);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][wid] = s;
+        sh[1][wid] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < RED_THREADS / 64; ++w) {
+            a += sh[0][w];
+            b += sh[1][w];
+        }
+        *out = a / b;
+    }
+}
+
+__global__ void k_fill_int(int* __restrict__ p, long long n, int v) {
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x)
+        p[c] = v;
+}
+
+PArgs particle_args(const lbm_ctx* c) {
+    const ParticleState* s = c->part;
+    PArgs a{};
+    a.nx = c->p.nx;
+    a.ny = c->p.ny;
+    a.nz = c->p.nz;
+    a.px = c->px;
+    a.pxy = c->pxy;
+    a.ps = c->ps;
+    a.snu = 1.0 / c->p.tau;
+    a.sq = 8.0 * (2.0 * c->p.tau - 1.0) / (8.0 * c->p.tau - 1.0);
+    a.scale = c->scale;
+    a.inv_scale = c->inv_scale;
+    a.meq12_rho = c->p.meq12_rho;
+    a.rho0 = c->p.rho0;
+    a.rho_solid = s->pp.rho_solid;
+    a.owner = s->owner[s->cur];
+    a.owner_new = s->owner[1 - s->cur];
+    a.xc = s->xc[s->cur];
+    a.U = s->U[s->cur];
+    a.Om = s->Om[s->cur];
+    a.xc_new = a.xc_out = s->xc[1 - s->cur];
+    a.U_new = a.U_out = s->U[1 - s->cur];
+    a.Om_new = a.Om_out = s->Om[1 - s->cur];
+    a.r = s->r;
+    a.rho_avg = s->scal;
+    a.rho = c->macro;
+    a.u = c->macro + c->n;
+    a.v = c->macro + 2 * c->n;
+    a.w = c->macro + 3 * c->n;
+    a.err = s->err;
+    a.np = s->np;
+    a.gravity = s->pp.gravity;
+    a.threshold_wall = s->pp.threshold_wall;
+    a.stiff_wall = s->pp.stiff_wall;
+    a.threshold_particle = s->pp.threshold_particle;
+    a.stiff_particle = s->pp.stiff_particle;
+    a.F = s->F;
+    a.T = s->T;
+    a.Ftot = s->Ftot;
+    a.fstream = s->fstream;
+    return a;
+}
+
+dim3 cell_grid(const lbm_ctx* c) {
+    return dim3((c->p.nx + c->block.x - 1) / c->block.x, (c->p.ny + c->block.y - 1) / c->block.y, c->p.nz);
+}
+
+void launch_fluid_mean(lbm_ctx* c, const int* owner, double* out) {
+    ParticleState* s = c->part;
+    hipLaunchKernelGGL(k_fluid_sum, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->macro, owner, c->n, s->red);
+    hipLaunchKernelGGL(k_fluid_mean, dim3(1), dim3(RED_THREADS), 0, c->stream, s->red, RED_BLOCKS, out);
+}
+
+template <class S, int COLL>
+void particle_launches(lbm_ctx* c) {
+    ParticleState* s = c->part;
+    const int np = s->np;
+    launch_fluid_mean(c, s->owner[s->cur], s->scal);
+    PArgs a = particle_args(c);
+    a.src = c->buf[c->cur];
+    a.dst = c->buf[1 - c->cur];
+    hipLaunchKernelGGL(k_particle_links<S>, dim3(np), dim3(LINK_THREADS), 0, c->stream, a);
+    hipLaunchKernelGGL((k_particle_step<S, COLL>), cell_grid(c), c->block, 0, c->stream, a);
+    hipLaunchKernelGGL(k_particle_update, dim3((np + 255) / 256), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_fill_int, dim3(2048), dim3(256), 0, c->stream, s->owner[1 - s->cur], c->n, -1);
+    hipLaunchKernelGGL(k_particle_mark, dim3(np), dim3(256), 0, c->stream, a, s->owner[1 - s->cur]);
+    launch_fluid_mean(c, s->owner[1 - s->cur], s->scal + 1);
+    hipLaunchKernelGGL((k_particle_refill<S, COLL>), dim3(np), dim3(256), 0, c->stream, a);
+    s->cur = 1 - s->cur;
+    c->cur = 1 - c->cur;
+}
+
+void free_state(ParticleState* s) {
+    for (int q = 0; q < 2; ++q) {
+        if (s->xc[q]) (void)hipFree(s->xc[q]);
+        if (s->U[q]) (void)hipFree(s->U[q]);
+        if (s->Om[q]) (void)hipFree(s->Om[q]);
+        if (s->owner[q]) (void)hipFree(s->owner[q]);
+    }
+    for (void* p : {(void*)s->r, (void*)s->F, (void*)s->T, (void*)s->Ftot, s->fstream, (void*)s->scal, (void*)s->red,
+                    (void*)s->err})
+        if (p) (void)hipFree(p);
+}
+
+}  // namespace
+
+void particles_release(lbm_ctx* c) {
+    if (!c->part) return;
+    free_state(c->part);
+    delete c->part;
+    c->part = nullptr;
+}
+
+int particles_step(lbm_ctx* c, int nsteps) {
+    const bool bgk = c->p.collision == LBM_BGK;
+    for (int q = 0; q < nsteps; ++q) {
+        if (c->p.storage == LBM_STORE_F32) {
+            if (bgk) particle_launches<StoreF32, COLL_BGK>(c);
+            else particle_launches<StoreF32, COLL_MRT>(c);
+        } else {
+            if (bgk) particle_launches<StoreF64, COLL_BGK>(c);
+            else particle_launches<StoreF64, COLL_MRT>(c);
+        }
+    }
+    int rc = lbm_check_launch();
+    if (rc) return rc;
+    int err = 0;
+    HIP_TRY(hipMemcpyAsync(&err, c->part->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (err == 1) return lbm_fail(LBM_ERR_PARTICLE, "particle-particle interpenetration (particle_force.F90:117-124)");
+    if (err == 2) return lbm_fail(LBM_ERR_PARTICLE, "a particle penetrates a wall (particle_force.F90:136-186)");
+    if (err == 3) return lbm_fail(LBM_ERR_PARTICLE, "wall distance q outside [0, 1] (particle_bounceback.F90:128-138)");
+    if (err == 4) return lbm_fail(LBM_ERR_PARTICLE, "no extrapolation direction for an uncovered node");
+    return LBM_OK;
+}
+
+extern "C" {
+
+int lbm_particles_init(lbm_ctx* c, const lbm_particle_params* pp, const double* centers, const double* radii,
+                       const double* U, const double* omega) {
+    if (!c || !pp || !centers || !radii) return lbm_fail(LBM_ERR_ARG, "lbm_particles_init: null argument");
+    if (pp->np < 1) return lbm_fail(LBM_ERR_ARG, "lbm_particles_init: np = %d", pp->np);
+    if (c->p.thermal || c->p.halo_faces || c->p.halo_loopback || c->p.u_lid != 0.0 ||
+        (c->p.storage != LBM_STORE_F64 && c->p.storage != LBM_STORE_F32))
+        return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_particles_init: needs an isothermal FP64/FP32 context with walls "
+                                             "at rest and no halo");
+    if (!(pp->stiff_wall > 0.0) || !(pp->stiff_particle > 0.0) || !(pp->threshold_wall > 0.0) ||
+        !(pp->threshold_particle > 0.0))
+        return lbm_fail(LBM_ERR_ARG, "lbm_particles_init: spring thresholds and stiffnesses must be positive");
+    // a bounding box must stay below 2^27 nodes (link list entries pack the
+    // node offset into 27 bits)
+    for (int q = 0; q < pp->np; ++q) {
+        if (!(radii[q] > 0.0 && radii[q] <= 250.0))
+            return lbm_fail(LBM_ERR_ARG, "lbm_particles_init: radius %d = %g outside (0, 250]", q, radii[q]);
+        for (int d = 0; d < 3; ++d)
+            if (!std::isfinite(centers[3 * q + d]))
+                return lbm_fail(LBM_ERR_ARG, "lbm_particles_init: centre %d is not finite", q);
+    }
+    particles_release(c);
+    ParticleState* s = new (std::nothrow) ParticleState();
+    if (!s) return lbm_fail(LBM_ERR_NOMEM, "lbm_particles_init: host allocation failed");
+    s->np = pp->np;
+    s->pp = *pp;
+    const size_t v3 = sizeof(double) * 3 * pp->np;
+    hipError_t e = hipSuccess;
+    for (int q = 0; q < 2; ++q) {
+        if (e == hipSuccess) e = hipMalloc(&s->xc[q], v3);
+        if (e == hipSuccess) e = hipMalloc(&s->U[q], v3);
+        if (e == hipSuccess) e = hipMalloc(&s->Om[q], v3);
+        if (e == hipSuccess) e = hipMalloc(&s->owner[q], sizeof(int) * c->n);
+    }
+    if (e == hipSuccess) e = hipMalloc(&s->r, sizeof(double) * pp->np);
+    if (e == hipSuccess) e = hipMalloc(&s->F, v3);
+    if (e == hipSuccess) e = hipMalloc(&s->T, v3);
+    if (e == hipSuccess) e = hipMalloc(&s->Ftot, v3);
+    if (e == hipSuccess) e = hipMalloc(&s->fstream, c->elem * (size_t)c->ps * Q);
+    if (e == hipSuccess) e = hipMalloc(&s->scal, sizeof(double) * 2);
+    if (e == hipSuccess) e = hipMalloc(&s->red, sizeof(double) * 2 * RED_BLOCKS);
+    if (e == hipSuccess) e = hipMalloc(&s->err, sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(s->xc[0], centers, v3, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->r, radii, sizeof(double) * pp->np, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = U ? hipMemcpy(s->U[0], U, v3, hipMemcpyHostToDevice) : hipMemset(s->U[0], 0, v3);
+    if (e == hipSuccess)
+        e = omega ? hipMemcpy(s->Om[0], omega, v3, hipMemcpyHostToDevice) : hipMemset(s->Om[0], 0, v3);
+    if (e == hipSuccess) e = hipMemset(s->F, 0, v3);
+    if (e == hipSuccess) e = hipMemset(s->T, 0, v3);
+    if (e == hipSuccess) e = hipMemset(s->Ftot, 0, v3);
+    if (e == hipSuccess) e = hipMemset(s->err, 0, sizeof(int));
+    if (e != hipSuccess) {
+        free_state(s);
+        delete s;
+        (void)hipGetLastError();
+        return lbm_fail(LBM_ERR_NOMEM, "lbm_particles_init: %s", hipGetErrorString(e));
+    }
+    c->part = s;
+    s->cur = 0;
+    c->cur = 0;
+    PArgs a = particle_args(c);
+    hipLaunchKernelGGL(k_fill_int, dim3(2048), dim3(256), 0, c->stream, s->owner[0], c->n, -1);
+    hipLaunchKernelGGL(k_particle_init_mark, dim3(pp->np), dim3(256), 0, c->stream, a, s->owner[0]);
+    a.dst = c->buf[0];
+    if (c->p.storage == LBM_STORE_F32) {
+        if (c->p.collision == LBM_BGK)
+            hipLaunchKernelGGL((k_particle_init_fields<StoreF32, COLL_BGK>), cell_grid(c), c->block, 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((k_particle_init_fields<StoreF32, COLL_MRT>), cell_grid(c), c->block, 0, c->stream, a);
+    } else {
+        if (c->p.collision == LBM_BGK)
+            hipLaunchKernelGGL((k_particle_init_fields<StoreF64, COLL_BGK>), cell_grid(c), c->block, 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((k_particle_init_fields<StoreF64, COLL_MRT>), cell_grid(c), c->block, 0, c->stream, a);
+    }
+    // residual history up = vp = 0: 2 n doubles of zero bits
+    hipLaunchKernelGGL(k_fill_int, dim3(2048), dim3(256), 0, c->stream, reinterpret_cast<int*>(c->prev), 4 * c->n, 0);
+    int rc = lbm_check_launch();
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->itc = 0;
+    c->initialised = true;
+    return LBM_OK;
+}
+
+int lbm_particles_get(lbm_ctx* c, double* centers, double* U, double* omega, double* F, double* T) {
+    if (!c || !c->part) return lbm_fail(LBM_ERR_STATE, "lbm_particles_get: no particles attached");
+    ParticleState* s = c->part;
+    const size_t v3 = sizeof(double) * 3 * s->np;
+    if (centers) HIP_TRY(hipMemcpyAsync(centers, s->xc[s->cur], v3, hipMemcpyDeviceToHost, c->stream));
+    if (U) HIP_TRY(hipMemcpyAsync(U, s->U[s->cur], v3, hipMemcpyDeviceToHost, c->stream));
+    if (omega) HIP_TRY(hipMemcpyAsync(omega, s->Om[s->cur], v3, hipMemcpyDeviceToHost, c->stream));
+    if (F) HIP_TRY(hipMemcpyAsync(F, s->Ftot, v3, hipMemcpyDeviceToHost, c->stream));
+    if (T) HIP_TRY(hipMemcpyAsync(T, s->T, v3, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_particles_map(lbm_ctx* c, int* owner) {
+    if (!c || !c->part || !owner) return lbm_fail(LBM_ERR_STATE, "lbm_particles_map: no particles attached");
+    HIP_TRY(hipMemcpyAsync(owner, c->part->owner[c->part->cur], sizeof(int) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+}  // extern "C"

@@ -34,7 +34,7 @@ EXPORTED_SYMBOLS = (
     "lbm_halo_unpack", "lbm_check_thermal", "lbm_nusselt", "lbm_download_temperature",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_initial", "lbm_batch_step", "lbm_batch_step_each",
     "lbm_batch_autotune", "lbm_batch_set_launch", "lbm_batch_synchronize", "lbm_batch_download_macro",
-    "lbm_batch_get_info",
+    "lbm_batch_get_info", "lbm_particles_init", "lbm_particles_get", "lbm_particles_map",
 )
 
 FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = range(6)
@@ -71,6 +71,16 @@ class LbmBatchInfo(ctypes.Structure):
         ("steps_done", ctypes.c_longlong), ("device_bytes", ctypes.c_longlong), ("bytes_per_lup", ctypes.c_double),
         ("threads", ctypes.c_int), ("blocks", ctypes.c_int), ("graph_steps", ctypes.c_int),
         ("stream", ctypes.c_void_p),
+    ]
+
+
+class LbmParticleParams(ctypes.Structure):
+    _fields_ = [
+        ("np", ctypes.c_int), ("reserved_i", ctypes.c_int),
+        ("rho_solid", ctypes.c_double), ("gravity", ctypes.c_double),
+        ("threshold_wall", ctypes.c_double), ("stiff_wall", ctypes.c_double),
+        ("threshold_particle", ctypes.c_double), ("stiff_particle", ctypes.c_double),
+        ("reserved_d", ctypes.c_double * 4),
     ]
 
 
@@ -120,6 +130,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lbm_batch_synchronize.argtypes = [vp]
     lib.lbm_batch_download_macro.argtypes = [vp, ip, dp, dp, dp, dp]
     lib.lbm_batch_get_info.argtypes = [vp, ctypes.POINTER(LbmBatchInfo)]
+    lib.lbm_particles_init.argtypes = [vp, ctypes.POINTER(LbmParticleParams), dp, dp, dp, dp]
+    lib.lbm_particles_get.argtypes = [vp, dp, dp, dp, dp, dp]
+    lib.lbm_particles_map.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     lib.lbm_status_string.argtypes = [ip]
     lib.lbm_status_string.restype = ctypes.c_char_p
     lib.lbm_last_error.restype = ctypes.c_char_p
@@ -199,6 +212,47 @@ def buoyancy_initial_temperature(nx, ny, nz, t_hot=0.5, t_cold=-0.5, seed=2024, 
     T[:, 0, :] = t_hot
     T[:, -1, :] = t_cold
     return T
+
+
+def random_spheres(nx: int, ny: int, nz: int, count: int = 2000, rmin: float = 4.0, rmax: float = 8.0,
+                   seed: int = 2025, gap: float = 1.0, max_tries: int = 200000):
+    """Seeded random sequential addition of non-overlapping spheres in an
+    nx x ny x nz box: radii uniform in [rmin, rmax], placed largest first at
+    uniform random centres (1-based node coordinates) keeping `gap` lattice
+    units from every other sphere and from the walls (the low wall of axis d
+    lies at coordinate 1, the high one at N, as particle_force.F90:128-186
+    places them).  Returns (centres (count, 3), radii (count,))."""
+    rng = np.random.default_rng(seed)
+    radii = np.sort(rng.uniform(rmin, rmax, count))[::-1].copy()
+    N = np.array([nx, ny, nz], dtype=np.float64)
+    cs = 2.0 * rmax + gap  # grid cell: no two touching candidates are two cells apart
+    ncell = np.maximum((N // cs).astype(int), 1)
+    grid: dict = {}
+    xc = np.zeros((count, 3))
+    offsets = [(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)]
+    for q, r in enumerate(radii):
+        lo, hi = 1.0 + r + gap, N - r - gap
+        if np.any(hi <= lo):
+            raise ValueError(f"sphere of radius {r} does not fit the box")
+        for _ in range(max_tries):
+            p = lo + rng.random(3) * (hi - lo)
+            cell = tuple(int(v) for v in np.minimum((p // cs).astype(int), ncell - 1))
+            clear = True
+            for o3 in offsets:
+                for o in grid.get((cell[0] + o3[0], cell[1] + o3[1], cell[2] + o3[2]), ()):
+                    d = xc[o] - p
+                    if d @ d < (r + radii[o] + gap) ** 2:
+                        clear = False
+                        break
+                if not clear:
+                    break
+            if clear:
+                xc[q] = p
+                grid.setdefault(cell, []).append(q)
+                break
+        else:
+            raise RuntimeError(f"could not place sphere {q} of {count} (packing too dense)")
+    return xc, radii
 
 
 @dataclass
@@ -333,6 +387,32 @@ class Cavity:
 
     def synchronize(self):
         self._call("lbm_synchronize", self._ctx)
+
+    def particles_init(self, centers, radii, rho_solid=1.5, gravity=1e-5, threshold_wall=4.0, stiff_wall=0.10,
+                       threshold_particle=4.0, stiff_particle=0.10, U=None, omega=None):
+        """Attach spheres (centres in 1-based node coordinates) to a fluid at rest."""
+        xc = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        r = np.ascontiguousarray(radii, dtype=np.float64).reshape(-1)
+        if r.shape[0] != xc.shape[0]:
+            raise ValueError("one radius per centre")
+        pp = LbmParticleParams(np=xc.shape[0], rho_solid=rho_solid, gravity=gravity, threshold_wall=threshold_wall,
+                               stiff_wall=stiff_wall, threshold_particle=threshold_particle,
+                               stiff_particle=stiff_particle)
+        vel = None if U is None else np.ascontiguousarray(U, dtype=np.float64).reshape(xc.shape)
+        om = None if omega is None else np.ascontiguousarray(omega, dtype=np.float64).reshape(xc.shape)
+        self._np = xc.shape[0]
+        self._call("lbm_particles_init", self._ctx, ctypes.byref(pp), _dp(xc), _dp(r), _dp(vel), _dp(om))
+
+    def particles(self):
+        """dict of np x 3 arrays: xc, U, Om, F (total force), T (hydrodynamic torque)."""
+        out = {k: np.empty((self._np, 3)) for k in ("xc", "U", "Om", "F", "T")}
+        self._call("lbm_particles_get", self._ctx, *[_dp(out[k]) for k in ("xc", "U", "Om", "F", "T")])
+        return out
+
+    def particle_map(self):
+        owner = np.empty(self.shape, dtype=np.int32)
+        self._call("lbm_particles_map", self._ctx, owner.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return owner
 
     def info(self) -> LbmInfo:
         inf = LbmInfo()
