@@ -1,15 +1,28 @@
-"""Benchmark of the D3Q19 lid-driven cavity step on one MI355X.
+"""Benchmark of the D3Q19 lattice-Boltzmann hot path on one MI355X.
 
-Workload: lid-driven cavity, D3Q19, 256^3 cells, Re = 1000, U0 = 0.1,
-tau = 3 U0 N / Re + 1/2 = 0.5768, MRT, reference initial state.  A step is
-the six-face + twelve-edge halo pack and unpack looped back onto the same
-GPU followed by one fused pull-scheme stream + bounce-back + macro + collide
-pass over the lattice (A-B buffers).  `value` is the plain FP64 path's MLUPS
-for that step with the state resident in HBM; the FP32- and FP16-stored
-variants are timed the same way and their relative L2 velocity error and
-mass drift against the FP64 run (same initial state, same step count) are
-reported beside them.  `roofline` is for the step kernel alone, timed over the
-same number of steps without the exchange.
+Default workload (the headline line): lid-driven cavity, D3Q19, 256^3 cells,
+Re = 1000, U0 = 0.1, tau = 3 U0 N / Re + 1/2 = 0.5768, MRT, reference initial
+state.  A step is the six-face + twelve-edge halo pack and unpack looped back
+onto the same GPU followed by one fused pull-scheme stream + bounce-back +
+macro + collide pass over the lattice (A-B buffers).  `value` is the plain
+FP64 path's MLUPS for that step with the state resident in HBM; the FP32- and
+FP16-stored variants are timed the same way and their relative L2 velocity
+error and mass drift against the FP64 run (same initial state, same step
+count) are reported beside them.  `roofline` is for the step kernel alone,
+timed over the same number of steps without the exchange.
+
+Other workloads (--workload), one JSON line each in the same format:
+  batch      64 lid-driven cavities of N in {32,48,64,80,96}^3, Re in
+             [100,1000], U0 in [0.05,0.1] (seeded), FP32 storage, one
+             persistent kernel replayed from 100-step hipGraphs, autotuned;
+             aggregate MLUPS beside launching the instances one at a time
+  particles  192^3 with 2000 rigid spheres of radius 4-8 at seeded random
+             non-overlapping positions, rho_s / rho_f = 1.5, g = 1e-5 along
+             -z, tau = 0.65, FP32 storage with FP64 force/torque sums; the map
+             is checked bit for bit and the trajectories against the FP64 path
+  thermal    buoyancy-driven cavity 128^3, D3Q19 MRT + Guo force and D3Q7
+             MRT temperature, Ra = 1e5, Pr = 0.71, walls at +-0.5, seeded
+             noise; Nusselt numbers after --accuracy-steps
 
 Prints ONE JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
@@ -27,37 +40,50 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "D3Q19 MLUPS + %HBM peak; FP16/FP32-store vs FP64 at stated L2 error"
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--n", type=int, default=256, help="cells per edge")
+    ap.add_argument("--workload", default="cavity", choices=["cavity", "batch", "particles", "thermal"])
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default per workload)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default per workload)")
+    ap.add_argument("--n", type=int, default=None, help="cells per edge (cavity 256, particles 192, thermal 128)")
     ap.add_argument("--collision", default="mrt", choices=["mrt", "bgk"])
-    ap.add_argument("--accuracy-steps", type=int, default=5000,
-                    help="total steps after which the reduced-storage fields are compared with FP64")
-    ap.add_argument("--variants", default="f32,f16", help="comma list of reduced storages to run ('' = none)")
+    ap.add_argument("--accuracy-steps", type=int, default=None,
+                    help="total steps after which reduced storage is compared with FP64 (default per workload)")
+    ap.add_argument("--variants", default="f32,f16", help="cavity: comma list of reduced storages ('' = none)")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle on a bounded sample")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--halo-loopback", type=int, default=1,
-                    help="1: pack/unpack the six faces and twelve edges every step, looped back")
+                    help="cavity: 1 = pack/unpack the six faces and twelve edges every step, looped back")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of this many MiB (counter calibration) and exit")
-    return ap.parse_args()
+    args = ap.parse_args()
+    defaults = {  # steps, warmup, n, accuracy steps
+        "cavity": (1000, 50, 256, 5000),
+        "batch": (1000, 200, 0, 0),
+        "particles": (1000, 20, 192, 5000),
+        "thermal": (2000, 50, 128, 20000),
+    }[args.workload]
+    args.steps = defaults[0] if args.steps is None else args.steps
+    args.warmup = defaults[1] if args.warmup is None else args.warmup
+    args.n = defaults[2] if args.n is None else args.n
+    args.accuracy_steps = defaults[3] if args.accuracy_steps is None else args.accuracy_steps
+    return args
 
 
-def timed_steps(cav, stream, args):
-    """W untimed steps, then K steps between HIP events on the library's stream."""
+def timed(step, stream, args):
+    """W untimed steps, then K steps between HIP events on `stream`."""
     from mglc_amd.hip import Event
 
-    cav.step(args.warmup)
+    step(args.warmup)
     stream.synchronize()
     e0, e1 = Event(), Event()
     t0 = time.perf_counter()
     e0.record(stream)
-    cav.step(args.steps)
+    step(args.steps)
     e1.record(stream)
     e1.synchronize()
     wall = time.perf_counter() - t0
@@ -65,53 +91,6 @@ def timed_steps(cav, stream, args):
     e0.close()
     e1.close()
     return ms, wall
-
-
-def run_storage(args, storage, stream):
-    from mglc_amd.lbm import Cavity, CavityConfig
-
-    n = args.n
-    base = dict(Reynolds=1000.0, U0=0.1, collision=args.collision, storage=storage)
-    with Cavity(CavityConfig(n, n, n, halo_loopback=args.halo_loopback, **base), stream=stream.handle.value) as cav:
-        cav.initial()
-        ms, wall = timed_steps(cav, stream, args)
-        info = cav.info()
-        done = args.warmup + args.steps
-        if args.accuracy_steps > done:
-            cav.step(args.accuracy_steps - done)
-        fields = cav.macro()
-        steps_total = max(done, args.accuracy_steps)
-    if args.halo_loopback:
-        # the step kernel alone, for its roofline
-        with Cavity(CavityConfig(n, n, n, **base), stream=stream.handle.value) as cav:
-            cav.initial()
-            ms_kernel, _ = timed_steps(cav, stream, args)
-    else:
-        ms_kernel = ms
-    cells = n ** 3
-    return dict(storage=storage, ms=ms, ms_kernel=ms_kernel, wall_s=wall,
-                mlups=cells * args.steps / (ms * 1e-3) / 1e6, bytes_per_lup=info.bytes_per_lup, fields=fields,
-                steps_total=steps_total)
-
-
-def cpu_baseline(args):
-    """The oracle (CPU restatement, OpenMP) on a bounded sample of the same
-    workload: 256^3 cells from the reference initial state, a few steps."""
-    from oracle import oracle as O
-    n = args.n
-    threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
-    tau = 0.1 * n / 1000.0 * 3.0 + 0.5
-    coll = O.MRT if args.collision == "mrt" else O.BGK
-    p = O.cavity_params(n, n, n, tau, 0.1, collision=coll)
-    steps = 1
-    O.cavity_run(p, steps, nthreads=threads)
-    per_step = max(O.last_loop_seconds(), 1e-3)
-    steps = int(max(1, min(200, 15.0 / per_step)))
-    O.cavity_run(p, steps, nthreads=threads)
-    sec = O.last_loop_seconds()
-    return {"value": n ** 3 * steps / sec / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"{n}^3 {args.collision.upper()} FP64 cavity, {steps} steps from the reference "
-                      f"initial state, oracle/lbm_oracle.c with {threads} OpenMP threads, loop time only"}
 
 
 def traffic_from_profiles(workload_key):
@@ -126,20 +105,76 @@ def traffic_from_profiles(workload_key):
         return None
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} refused: this build runs the cavity on one GPU "
-              "(the multi-GPU halo path is design only)", file=sys.stderr)
-        sys.exit(2)
-    if args.copy_calibration:
-        from mglc_amd.hip import device_copy_calibration
-        device_copy_calibration(args.copy_calibration << 20, 5)
-        print(json.dumps({"copy_calibration_bytes": args.copy_calibration << 20, "reps": 5}))
-        return
-    from mglc_amd.hip import Stream
+def roofline(bytes_per_launch, ms_per_launch, traffic=None):
+    achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
 
-    stream = Stream()
+
+def line_head(args, value, ms_step, dtype, data, config):
+    return {"metric": METRIC, "value": round(value, 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": data, "config": config}
+
+
+def oracle_rate(run, cells, budget_s=15.0, max_steps=200):
+    """MLUPS of an oracle run callable run(steps) -> seconds, on a sample of
+    about budget_s seconds."""
+    per = max(run(1), 1e-3)
+    steps = int(max(1, min(max_steps, budget_s / per)))
+    sec = run(steps)
+    return cells * steps / sec / 1e6, steps
+
+
+# ---- lid-driven cavity (headline) ------------------------------------------
+def run_storage(args, storage, stream):
+    from mglc_amd.lbm import Cavity, CavityConfig
+
+    n = args.n
+    base = dict(Reynolds=1000.0, U0=0.1, collision=args.collision, storage=storage)
+    with Cavity(CavityConfig(n, n, n, halo_loopback=args.halo_loopback, **base), stream=stream.handle.value) as cav:
+        cav.initial()
+        ms, wall = timed(cav.step, stream, args)
+        info = cav.info()
+        done = args.warmup + args.steps
+        if args.accuracy_steps > done:
+            cav.step(args.accuracy_steps - done)
+        fields = cav.macro()
+        steps_total = max(done, args.accuracy_steps)
+    if args.halo_loopback:
+        # the step kernel alone, for its roofline
+        with Cavity(CavityConfig(n, n, n, **base), stream=stream.handle.value) as cav:
+            cav.initial()
+            ms_kernel, _ = timed(cav.step, stream, args)
+    else:
+        ms_kernel = ms
+    cells = n ** 3
+    return dict(storage=storage, ms=ms, ms_kernel=ms_kernel, wall_s=wall,
+                mlups=cells * args.steps / (ms * 1e-3) / 1e6, bytes_per_lup=info.bytes_per_lup, fields=fields,
+                steps_total=steps_total)
+
+
+def cpu_baseline_cavity(args):
+    """The oracle (CPU restatement, OpenMP) on a bounded sample of the same
+    workload: 256^3 cells from the reference initial state, a few steps."""
+    from oracle import oracle as O
+    n = args.n
+    threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+    tau = 0.1 * n / 1000.0 * 3.0 + 0.5
+    coll = O.MRT if args.collision == "mrt" else O.BGK
+    p = O.cavity_params(n, n, n, tau, 0.1, collision=coll)
+
+    def run(steps):
+        O.cavity_run(p, steps, nthreads=threads)
+        return O.last_loop_seconds()
+
+    rate, steps = oracle_rate(run, n ** 3)
+    return {"value": rate, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{n}^3 {args.collision.upper()} FP64 cavity, {steps} steps from the reference "
+                      f"initial state, oracle/lbm_oracle.c with {threads} OpenMP threads, loop time only"}
+
+
+def bench_cavity(args, stream):
     primary = run_storage(args, "f64", stream)
     variants = {}
     for st in [s for s in args.variants.split(",") if s]:
@@ -156,39 +191,229 @@ def main():
                         "speedup_vs_f64": round(r["mlups"] / primary["mlups"], 3),
                         "rel_l2_velocity_vs_f64": l2, "mass_drift_vs_f64": drift,
                         "compared_at_step": r["steps_total"]}
-    stream.close()
-
-    ms_step = primary["ms"] / args.steps
-    bpl = primary["bytes_per_lup"]
-    achieved = bpl * args.n ** 3 / (primary["ms_kernel"] / args.steps * 1e-3) / 1e9
     m0 = float(primary["fields"][0].size)
     workload = f"lid-driven cavity D3Q19 {args.n}^3 Re=1000 U0=0.1 {args.collision.upper()}"
-    line = {
-        "metric": "D3Q19 MLUPS + %HBM peak; FP16/FP32-store vs FP64 at stated L2 error",
-        "value": round(primary["mlups"], 1),
-        "unit": "MLUPS",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (reference initial state: rho=1, u=0, lid layer u=U0)",
-        "config": {"workload": workload, "cells": args.n ** 3, "collision": args.collision,
-                   "storage": "f64", "pattern": "A-B pull", "kernel": "k_cavity_step",
-                   "halo": "6 faces x 5 pops + 12 edges, pack/unpack looped back every step"
-                   if args.halo_loopback else "none",
-                   "kernel_ms_per_step": round(primary["ms_kernel"] / args.steps, 4)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(f"{args.n}^3 {args.collision} f64")},
-        "mass_drift_f64": abs(float(primary["fields"][0].sum()) - m0) / m0,
-        "variants": variants,
-    }
+    config = {"workload": workload, "cells": args.n ** 3, "collision": args.collision,
+              "storage": "f64", "pattern": "A-B pull", "kernel": "k_cavity_step",
+              "halo": "6 faces x 5 pops + 12 edges, pack/unpack looped back every step"
+              if args.halo_loopback else "none",
+              "kernel_ms_per_step": round(primary["ms_kernel"] / args.steps, 4)}
+    line = line_head(args, primary["mlups"], primary["ms"] / args.steps, "f64",
+                     "synthetic (reference initial state: rho=1, u=0, lid layer u=U0)", config)
+    line["roofline"] = roofline(primary["bytes_per_lup"] * args.n ** 3, primary["ms_kernel"] / args.steps,
+                                traffic_from_profiles(f"{args.n}^3 {args.collision} f64"))
+    line["mass_drift_f64"] = abs(float(primary["fields"][0].sum()) - m0) / m0
+    line["variants"] = variants
     if args.cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args)
+        line["cpu_baseline"] = cpu_baseline_cavity(args)
+    return line
+
+
+# ---- batched cavities --------------------------------------------------------
+def bench_batch(args, stream_unused):
+    from mglc_amd.hip import Stream
+    from mglc_amd.lbm import Batch, batch_configs
+
+    cfgs = batch_configs(64, seed=12345, storage="f32", collision=args.collision)
+    cells = sum(c.nx * c.ny * c.nz for c in cfgs)
+    args.steps = max(100, args.steps // 100 * 100)   # whole graph replays
+    args.warmup = max(100, args.warmup // 100 * 100)
+    each_steps = 100
+    with Batch(cfgs) as b:
+        b.initial()
+        threads, blocks = b.autotune(trial_steps=4)
+        info = b.info()
+        s = Stream(borrowed=info.stream)
+        ms, wall = timed(b.step, s, args)
+        a2 = argparse.Namespace(warmup=2, steps=each_steps)
+        ms_each, _ = timed(b.step_each, s, a2)
+        info = b.info()
+        s.close()
+    mlups = cells * args.steps / (ms * 1e-3) / 1e6
+    mlups_each = cells * each_steps / (ms_each * 1e-3) / 1e6
+    sizes = sorted({c.nx for c in cfgs})
+    config = {"workload": f"64 lid-driven cavities D3Q19, N in {{32,48,64,80,96}}^3 (seed 12345), Re U[100,1000], "
+                          f"U0 U[0.05,0.1], {args.collision.upper()}, FP32 storage",
+              "cells": cells, "instances": len(cfgs), "sizes_present": sizes, "storage": "f32",
+              "kernel": "k_batch_step (persistent, 100-step hipGraph replays)", "threads": threads,
+              "blocks": blocks, "graph_steps": info.graph_steps}
+    line = line_head(args, mlups, ms / args.steps, "f32", "synthetic (reference initial state per instance)", config)
+    # kernel time per launch: the graph holds one step kernel and one 1-thread
+    # step-counter kernel per step
+    line["roofline"] = roofline(info.bytes_per_lup * cells, ms / args.steps, traffic_from_profiles("batch64 f32"))
+    line["one_at_a_time"] = {"mlups": round(mlups_each, 1), "ms_per_step": round(ms_each / each_steps, 4),
+                             "launches_per_step": len(cfgs), "speedup_batched": round(mlups / mlups_each, 3)}
+    if args.cpu_baseline:
+        from oracle import oracle as O
+        threads_c = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        params = [O.cavity_params(c.nx, c.ny, c.nz, c.tau(), c.U0, collision=O.MRT if args.collision == "mrt"
+                                  else O.BGK) for c in cfgs]
+
+        def run(steps):
+            tot = 0.0
+            for p in params:
+                O.cavity_run(p, steps, nthreads=threads_c)
+                tot += O.last_loop_seconds()
+            return tot
+
+        rate, steps = oracle_rate(run, cells, budget_s=15.0, max_steps=50)
+        line["cpu_baseline"] = {"value": rate, "unit": "MLUPS", "cores": threads_c, "kind": "port",
+                                "sample": f"the 64 instances in FP64, {steps} steps each, one after another, "
+                                          f"oracle/lbm_oracle.c with {threads_c} OpenMP threads, loop time only"}
+    return line
+
+
+# ---- particle-laden flow -----------------------------------------------------
+def node_map(shape, xc, r):
+    """Owner map by direct enumeration: node (i,j,k) at 1-based coordinates,
+    squared distance summed as ((dx*dx + dy*dy) + dz*dz), the last sphere
+    claiming a node keeps it."""
+    nz, ny, nx = shape
+    owner = -np.ones(shape, dtype=np.int32)
+    for q in range(len(r)):
+        lo = np.maximum(np.floor(xc[q] - r[q]).astype(int) - 2, 0)
+        hi = np.minimum(np.ceil(xc[q] + r[q]).astype(int) + 1, np.array([nx, ny, nz]))
+        k, j, i = np.meshgrid(np.arange(lo[2], hi[2]), np.arange(lo[1], hi[1]), np.arange(lo[0], hi[0]),
+                              indexing="ij")
+        dx, dy, dz = (i + 1.0) - xc[q, 0], (j + 1.0) - xc[q, 1], (k + 1.0) - xc[q, 2]
+        sel = (dx * dx + dy * dy) + dz * dz <= r[q] * r[q]
+        owner[k[sel], j[sel], i[sel]] = q
+    return owner
+
+
+def bench_particles(args, stream):
+    from mglc_amd.lbm import Cavity, CavityConfig, random_spheres
+
+    n, count, tau, rho_s, g = args.n, 2000, 0.65, 1.5, 1e-5
+    xc0, r = random_spheres(n, n, n, count=count, rmin=4.0, rmax=8.0, seed=2025)
+
+    def cfg(storage):
+        return CavityConfig(n, n, n, U0=0.0, tauf=tau, meq12_rho=1, storage=storage, collision=args.collision)
+
+    out = {}
+    for storage in ("f32", "f64"):
+        with Cavity(cfg(storage), stream=stream.handle.value) as cav:
+            cav.particles_init(xc0, r, rho_solid=rho_s, gravity=g)
+            fluid_nodes = int((cav.particle_map() < 0).sum())
+            ms, _ = timed(cav.step, stream, args)
+            done = args.warmup + args.steps
+            if args.accuracy_steps > done:
+                cav.step(args.accuracy_steps - done)
+            st = cav.particles()
+            owner = cav.particle_map()
+            info = cav.info()
+        out[storage] = dict(ms=ms, st=st, owner=owner, fluid0=fluid_nodes, bpl=info.bytes_per_lup)
+    f32, f64 = out["f32"], out["f64"]
+    steps_total = max(args.warmup + args.steps, args.accuracy_steps)
+    disp = np.linalg.norm(f64["st"]["xc"] - xc0, axis=1)
+    dev = np.linalg.norm(f32["st"]["xc"] - f64["st"]["xc"], axis=1)
+    map_exact = bool(np.array_equal(f32["owner"], node_map(f32["owner"].shape, f32["st"]["xc"], r)) and
+                     np.array_equal(f64["owner"], node_map(f64["owner"].shape, f64["st"]["xc"], r)))
+    cells = n ** 3
+    mlups = cells * args.steps / (f32["ms"] * 1e-3) / 1e6
+    # algorithmic bytes of a step: every fluid node reads and writes its 19
+    # populations, writes rho, u, v, w (FP64) and reads its owner entry
+    step_bytes = f32["fluid0"] * (f32["bpl"] + 4 * 8 + 4)
+    config = {"workload": f"particle-laden D3Q19 {n}^3, {count} spheres r in [4,8] (seed 2025), rho_s/rho_f=1.5, "
+                          f"g=1e-5 along -z, tau={tau}, {args.collision.upper()}, FP32 storage",
+              "cells": cells, "fluid_nodes": f32["fluid0"], "spheres": count, "storage": "f32",
+              "kernel": "particle step: links + fluid + update + map + refill kernels",
+              "fluid_mlups": round(f32["fluid0"] * args.steps / (f32["ms"] * 1e-3) / 1e6, 1)}
+    line = line_head(args, mlups, f32["ms"] / args.steps, "f32",
+                     "synthetic (seeded random sphere packing, fluid at rest)", config)
+    line["roofline"] = roofline(step_bytes, f32["ms"] / args.steps, traffic_from_profiles(f"particles{n} f32"))
+    line["fp64"] = {"mlups": round(cells * args.steps / (f64["ms"] * 1e-3) / 1e6, 1),
+                    "ms_per_step": round(f64["ms"] / args.steps, 4)}
+    line["accuracy"] = {"compared_at_step": steps_total, "map_bit_exact": map_exact,
+                        "max_displacement_f64": float(disp.max()),
+                        "max_trajectory_deviation_f32_vs_f64": float(dev.max()),
+                        "rel_trajectory_deviation": float(dev.max() / max(disp.max(), 1e-300)),
+                        "rel_velocity_deviation": float(np.abs(f32["st"]["U"] - f64["st"]["U"]).max() /
+                                                        max(np.abs(f64["st"]["U"]).max(), 1e-300))}
+    if args.cpu_baseline:
+        from oracle import oracle as O
+        p = O.particle_params(n, n, n, count, tau, rho_solid=rho_s, gravity=g,
+                              collision=O.MRT if args.collision == "mrt" else O.BGK)
+
+        def run(steps):
+            t0 = time.perf_counter()
+            O.particle_run(p, steps, xc0, r)
+            return time.perf_counter() - t0
+
+        t_init = run(0)
+        steps = 2
+        sec = run(steps) - t_init
+        line["cpu_baseline"] = {"value": cells * steps / sec / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+                                "sample": f"the same {n}^3 / {count}-sphere case in FP64, {steps} steps, "
+                                          "oracle/particle_oracle.c single-threaded, set-up time subtracted"}
+    return line
+
+
+# ---- buoyancy-driven cavity ----------------------------------------------------
+def bench_thermal(args, stream):
+    from mglc_amd.lbm import Cavity, ThermalConfig, buoyancy_initial_temperature
+
+    n = args.n
+    cfg = ThermalConfig(n, n, n, Rayleigh=1e5, Prandtl=0.71, Thot=0.5, Tcold=-0.5, thermal_axis=1)
+    T0 = buoyancy_initial_temperature(n, n, n, seed=2024)
+    z = np.zeros_like(T0)
+    with Cavity(cfg, stream=stream.handle.value) as cav:
+        cav.load_fields(np.ones_like(T0), z, z, z, T0)
+        ms, _ = timed(cav.step, stream, args)
+        done = args.warmup + args.steps
+        if args.accuracy_steps > done:
+            cav.step(args.accuracy_steps - done)
+        nu_vol, nu_wall = cav.nusselt()
+        info = cav.info()
+    cells = n ** 3
+    mlups = cells * args.steps / (ms * 1e-3) / 1e6
+    # k_thermal_step: 19 + 7 FP64 populations read and written, Fz read and written
+    bpl = 2 * (19 + 7) * 8 + 2 * 8
+    config = {"workload": f"buoyancy-driven cavity D3Q19+D3Q7 {n}^3 Ra=1e5 Pr=0.71 walls +-0.5, MRT, FP64",
+              "cells": cells, "storage": "f64", "kernel": "k_thermal_step", "bytes_per_lup": bpl}
+    line = line_head(args, mlups, ms / args.steps, "f64",
+                     "synthetic (rho=1, u=0, T=0 plus seeded noise +-1e-4, held wall layers)", config)
+    line["roofline"] = roofline(bpl * cells, ms / args.steps, traffic_from_profiles(f"thermal{n} f64"))
+    line["nusselt"] = {"at_step": max(args.warmup + args.steps, args.accuracy_steps), "nu_volume": nu_vol,
+                       "nu_wall": nu_wall}
+    if args.cpu_baseline:
+        from oracle import oracle as O
+        d = cfg.derived()
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        p = O.thermal_params(n, n, n, d["tau"], d["para_a"], d["g_beta"], 0.0, 0.5, -0.5, 0.0, tconst_axis=1)
+        fields = (np.ones_like(T0), z, z, z, T0)
+
+        def run(steps):
+            t0 = time.perf_counter()
+            O.thermal_run(p, steps, fields=fields, nthreads=threads)
+            return time.perf_counter() - t0
+
+        t_init = run(0)
+        rate, steps = oracle_rate(lambda s: max(run(s) - t_init, 1e-6), cells)
+        line["cpu_baseline"] = {"value": rate, "unit": "MLUPS", "cores": threads, "kind": "port",
+                                "sample": f"the same {n}^3 case in FP64, {steps} steps, oracle/thermal_oracle.c "
+                                          f"with {threads} OpenMP threads, set-up time subtracted"}
+    return line
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} refused: this build runs on one GPU "
+              "(the multi-GPU halo path is design only)", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        from mglc_amd.hip import device_copy_calibration
+        device_copy_calibration(args.copy_calibration << 20, 5)
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration << 20, "reps": 5}))
+        return
+    from mglc_amd.hip import Stream
+
+    stream = Stream()
+    fn = {"cavity": bench_cavity, "batch": bench_batch, "particles": bench_particles,
+          "thermal": bench_thermal}[args.workload]
+    line = fn(args, stream)
+    stream.close()
     print(json.dumps(line), flush=True)
 
 

@@ -222,6 +222,10 @@ typedef struct lbm_particle_params {
  * FP64/FP32 context with u_lid = 0 and no halo faces. */
 int lbm_particles_init(lbm_ctx *ctx, const lbm_particle_params *pp, const double *centers,
                        const double *radii, const double *U, const double *omega);
+/* A step call that meets a stop condition returns LBM_ERR_PARTICLE naming
+ * the first condition met and its step; the context then refuses further
+ * steps (the fields and sphere state are those of an unspecified point of the
+ * stopping step) until lbm_particles_init is called again. */
 /* Sphere state of the last step (each np*3; any pointer may be NULL):
  * centre, velocity, angular velocity, total force (hydrodynamic + springs +
  * buoyancy) and hydrodynamic torque. */

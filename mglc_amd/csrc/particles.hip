@@ -51,7 +51,8 @@ struct ParticleState {
     void* fstream = nullptr; // link values: f(abar, x) on links out of a solid (storage type, 19 ps)
     double* scal = nullptr;  // [0] rhoAvg of the streaming, [1] rhoAvg of the refill
     double* red = nullptr;   // reduction partials
-    int* err = nullptr;      // device stop code (0 = running)
+    int* err = nullptr;      // device stop record {code, step}; code 0 = running
+    bool stopped = false;    // a stop was reported: no further steps
 };
 
 namespace {
@@ -75,8 +76,8 @@ struct PArgs {
     const double* r;
     const double* rho_avg;  // device scalars: [0] streaming, [1] refill
     double *rho, *u, *v, *w;
-    int* err;
-    unsigned seed;
+    int* err;  // {code, step}: first stop wins
+    int step;  // number of the step being made
     int np;
     // sphere update
     double gravity, threshold_wall, stiff_wall, threshold_particle, stiff_particle;
@@ -127,6 +128,15 @@ __device__ __forceinline__ void body_velocity(const double* xc, const double* U,
     uy = U[3 * id + 1] + Om[3 * id + 2] * rx - Om[3 * id] * rz;
     uz = U[3 * id + 2] + Om[3 * id] * ry - Om[3 * id + 1] * rx;
 }
+
+// Record a stop: the first code raised in a step call wins, with its step.
+__device__ __forceinline__ void stop(const PArgs& a, int code) {
+    if (atomicCAS(a.err, 0, code) == 0) atomicExch(a.err + 1, a.step);
+}
+
+// Every particle kernel returns at once after a stop, so the state stays that
+// of the stopping step's launches.
+__device__ __forceinline__ bool stopped(const PArgs& a) { return *(volatile const int*)a.err != 0; }
 
 __device__ __forceinline__ bool in_box(const PArgs& a, int i, int j, int k) {
     return (unsigned)i < (unsigned)a.nx && (unsigned)j < (unsigned)a.ny && (unsigned)k < (unsigned)a.nz;
@@ -185,6 +195,7 @@ template <class S>
 __global__ __launch_bounds__(LINK_THREADS) void k_particle_links(const PArgs a) {
     using T = typename S::T;
     using R = typename S::Real;
+    if (stopped(a)) return;
     const int id = blockIdx.x;
     int lo[3], hi[3];
     sphere_box(a, a.xc, id, 1, lo, hi);
@@ -245,7 +256,7 @@ __global__ __launch_bounds__(LINK_THREADS) void k_particle_links(const PArgs a) 
             const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
             double x0, y0, z0, qq;
             if (!calq(cx, cy, cz, rad, i + 1.0, j + 1.0, k + 1.0, EX[al], EY[al], EZ[al], x0, y0, z0, qq))
-                atomicMax(a.err, 3);
+                stop(a, 3);
             double uwx, uwy, uwz;
             body_velocity(a.xc, a.U, a.Om, id, x0, y0, z0, uwx, uwy, uwz);
             const double eu = EX[bd] * uwx + EY[bd] * uwy + EZ[bd] * uwz;
@@ -305,7 +316,7 @@ __global__ __launch_bounds__(256) void k_particle_step(const PArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
-    if (i >= a.nx || j >= a.ny) return;
+    if (i >= a.nx || j >= a.ny || stopped(a)) return;
     const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
     if (a.owner[m] >= 0) return;
     const int c = i + a.px * (j + a.ny * k);
@@ -356,7 +367,7 @@ __device__ double streamed_pop(const PArgs& a, const typename S::T* src, const t
 
 __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.np) return;
+    if (p >= a.np || stopped(a)) return;
     const double rho_avg = a.rho_avg[0];
     const double rp = a.r[p];
     const double V = 4.0 / 3.0 * PI * rp * rp * rp;
@@ -370,7 +381,7 @@ __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
         const double rs = rp + a.r[q];
         if (dij >= rs + a.threshold_particle) continue;
         if (dij < rs) {
-            atomicMax(a.err, 1);
+            stop(a, 1);
             continue;
         }
         const double g = (dij - rs - a.threshold_particle) / a.threshold_particle;
@@ -384,7 +395,7 @@ __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
     for (int d = 0; d < 3; ++d) {
         const double lo = a.xc[3 * p + d] - rp - 1.0;
         const double hi = N[d] - a.xc[3 * p + d] - rp;
-        if (lo < 0.0 || hi < 0.0) atomicMax(a.err, 2);
+        if (lo < 0.0 || hi < 0.0) stop(a, 2);
         if (lo < a.threshold_wall) {
             const double g = (lo - a.threshold_wall) / a.threshold_wall;
             fs[d] += scale_w * (g * g);
@@ -409,6 +420,7 @@ __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
 
 // New map from the updated spheres; solids at rest with rho_s.
 __global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner_out) {
+    if (stopped(a)) return;
     const int id = blockIdx.x;
     int lo[3], hi[3];
     sphere_box(a, a.xc_new, id, 1, lo, hi);
@@ -483,6 +495,7 @@ template <class S, int COLL>
 __global__ __launch_bounds__(256) void k_particle_refill(const PArgs a) {
     using T = typename S::T;
     using R = typename S::Real;
+    if (stopped(a)) return;
     const int id = blockIdx.x;
     int lo[3], hi[3];
     sphere_box(a, a.xc, id, 1, lo, hi);  // old position
@@ -509,7 +522,7 @@ __global__ __launch_bounds__(256) void k_particle_refill(const PArgs a) {
             }
         }
         if (ec == 0) {
-            atomicMax(a.err, 4);
+            stop(a, 4);
             continue;
         }
         double fr[Q];
@@ -665,11 +678,12 @@ void launch_fluid_mean(lbm_ctx* c, const int* owner, double* out) {
 }
 
 template <class S, int COLL>
-void particle_launches(lbm_ctx* c) {
+void particle_launches(lbm_ctx* c, int step) {
     ParticleState* s = c->part;
     const int np = s->np;
     launch_fluid_mean(c, s->owner[s->cur], s->scal);
     PArgs a = particle_args(c);
+    a.step = step;
     a.src = c->buf[c->cur];
     a.dst = c->buf[1 - c->cur];
     hipLaunchKernelGGL(k_particle_links<S>, dim3(np), dim3(LINK_THREADS), 0, c->stream, a);
@@ -705,26 +719,33 @@ void particles_release(lbm_ctx* c) {
 }
 
 int particles_step(lbm_ctx* c, int nsteps) {
+    ParticleState* s = c->part;
+    if (s->stopped) return lbm_fail(LBM_ERR_STATE, "lbm_step: the particle run has stopped; attach the spheres again");
     const bool bgk = c->p.collision == LBM_BGK;
     for (int q = 0; q < nsteps; ++q) {
+        const int step = (int)(c->itc + q + 1);
         if (c->p.storage == LBM_STORE_F32) {
-            if (bgk) particle_launches<StoreF32, COLL_BGK>(c);
-            else particle_launches<StoreF32, COLL_MRT>(c);
+            if (bgk) particle_launches<StoreF32, COLL_BGK>(c, step);
+            else particle_launches<StoreF32, COLL_MRT>(c, step);
         } else {
-            if (bgk) particle_launches<StoreF64, COLL_BGK>(c);
-            else particle_launches<StoreF64, COLL_MRT>(c);
+            if (bgk) particle_launches<StoreF64, COLL_BGK>(c, step);
+            else particle_launches<StoreF64, COLL_MRT>(c, step);
         }
     }
     int rc = lbm_check_launch();
     if (rc) return rc;
-    int err = 0;
-    HIP_TRY(hipMemcpyAsync(&err, c->part->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    int err[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(err, s->err, sizeof err, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (err == 1) return lbm_fail(LBM_ERR_PARTICLE, "particle-particle interpenetration (particle_force.F90:117-124)");
-    if (err == 2) return lbm_fail(LBM_ERR_PARTICLE, "a particle penetrates a wall (particle_force.F90:136-186)");
-    if (err == 3) return lbm_fail(LBM_ERR_PARTICLE, "wall distance q outside [0, 1] (particle_bounceback.F90:128-138)");
-    if (err == 4) return lbm_fail(LBM_ERR_PARTICLE, "no extrapolation direction for an uncovered node");
-    return LBM_OK;
+    if (err[0] == 0) return LBM_OK;
+    s->stopped = true;
+    static const char* const why[] = {
+        "", "particle-particle interpenetration (particle_force.F90:117-124)",
+        "a particle penetrates a wall (particle_force.F90:136-186)",
+        "wall distance q outside [0, 1] (particle_bounceback.F90:128-138)",
+        "no extrapolation direction for an uncovered node (non-finite sphere state)"};
+    return lbm_fail(LBM_ERR_PARTICLE, "particle run stopped at step %d: %s", err[1],
+                    err[0] >= 1 && err[0] <= 4 ? why[err[0]] : "unknown stop code");
 }
 
 extern "C" {
@@ -769,7 +790,7 @@ int lbm_particles_init(lbm_ctx* c, const lbm_particle_params* pp, const double* 
     if (e == hipSuccess) e = hipMalloc(&s->fstream, c->elem * (size_t)c->ps * Q);
     if (e == hipSuccess) e = hipMalloc(&s->scal, sizeof(double) * 2);
     if (e == hipSuccess) e = hipMalloc(&s->red, sizeof(double) * 2 * RED_BLOCKS);
-    if (e == hipSuccess) e = hipMalloc(&s->err, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&s->err, 2 * sizeof(int));
     if (e == hipSuccess) e = hipMemcpy(s->xc[0], centers, v3, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->r, radii, sizeof(double) * pp->np, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = U ? hipMemcpy(s->U[0], U, v3, hipMemcpyHostToDevice) : hipMemset(s->U[0], 0, v3);
@@ -778,7 +799,7 @@ int lbm_particles_init(lbm_ctx* c, const lbm_particle_params* pp, const double* 
     if (e == hipSuccess) e = hipMemset(s->F, 0, v3);
     if (e == hipSuccess) e = hipMemset(s->T, 0, v3);
     if (e == hipSuccess) e = hipMemset(s->Ftot, 0, v3);
-    if (e == hipSuccess) e = hipMemset(s->err, 0, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(s->err, 0, 2 * sizeof(int));
     if (e != hipSuccess) {
         free_state(s);
         delete s;

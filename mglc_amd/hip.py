@@ -40,17 +40,23 @@ def _ok(rc, what):
 
 
 class Stream:
-    def __init__(self):
+    def __init__(self, borrowed: int | None = None):
+        """A new stream, or (borrowed = a hipStream_t address) a reference to a
+        stream owned elsewhere, which close() leaves alone."""
         self.handle = ctypes.c_void_p()
-        _ok(runtime().hipStreamCreate(ctypes.byref(self.handle)), "hipStreamCreate")
+        self._owned = borrowed is None
+        if self._owned:
+            _ok(runtime().hipStreamCreate(ctypes.byref(self.handle)), "hipStreamCreate")
+        else:
+            self.handle = ctypes.c_void_p(borrowed)
 
     def synchronize(self):
         _ok(runtime().hipStreamSynchronize(self.handle), "hipStreamSynchronize")
 
     def close(self):
-        if self.handle:
+        if self.handle and self._owned:
             runtime().hipStreamDestroy(self.handle)
-            self.handle = ctypes.c_void_p()
+        self.handle = ctypes.c_void_p()
 
 
 class Event:

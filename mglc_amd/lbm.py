@@ -215,13 +215,19 @@ def buoyancy_initial_temperature(nx, ny, nz, t_hot=0.5, t_cold=-0.5, seed=2024, 
 
 
 def random_spheres(nx: int, ny: int, nz: int, count: int = 2000, rmin: float = 4.0, rmax: float = 8.0,
-                   seed: int = 2025, gap: float = 1.0, max_tries: int = 200000):
-    """Seeded random sequential addition of non-overlapping spheres in an
-    nx x ny x nz box: radii uniform in [rmin, rmax], placed largest first at
-    uniform random centres (1-based node coordinates) keeping `gap` lattice
-    units from every other sphere and from the walls (the low wall of axis d
-    lies at coordinate 1, the high one at N, as particle_force.F90:128-186
-    places them).  Returns (centres (count, 3), radii (count,))."""
+                   seed: int = 2025, gap: float = 1.0, min_gap: float | None = 2.5, max_tries: int = 200000,
+                   relax_iters: int = 5000):
+    """Seeded random non-overlapping spheres in an nx x ny x nz box.
+
+    Radii are uniform in [rmin, rmax]; random sequential addition places them
+    largest first at uniform random centres (1-based node coordinates)
+    keeping `gap` lattice units from every other sphere and from the walls
+    (the low wall of axis d lies at coordinate 1, the high one at N, as
+    particle_force.F90:128-186 places them).  With min_gap set, a
+    deterministic relaxation then pushes overlapping pairs (and spheres too
+    close to a wall) apart until every gap is at least min_gap, so that the
+    spring forces of the particle scheme (active below 4 lattice units) start
+    weak.  Returns (centres (count, 3), radii (count,))."""
     rng = np.random.default_rng(seed)
     radii = np.sort(rng.uniform(rmin, rmax, count))[::-1].copy()
     N = np.array([nx, ny, nz], dtype=np.float64)
@@ -252,7 +258,40 @@ def random_spheres(nx: int, ny: int, nz: int, count: int = 2000, rmin: float = 4
                 break
         else:
             raise RuntimeError(f"could not place sphere {q} of {count} (packing too dense)")
+    if min_gap is not None and min_gap > gap:
+        xc = _relax_spheres(xc, radii, N, min_gap, relax_iters)
     return xc, radii
+
+
+def _relax_spheres(xc, r, N, min_gap, iters, step=0.6):
+    """Push pairs closer than min_gap (and spheres closer than min_gap to a
+    wall) apart, each by a fraction of its overlap, until none is left.
+    Pairs are visited in sorted order, so the result is reproducible."""
+    from scipy.spatial import cKDTree
+
+    slack = 0.05  # pushes aim this far beyond min_gap, so the iteration ends in finitely many steps
+    target = min_gap + slack
+    xc = xc.copy()
+    for _ in range(iters):
+        pairs = cKDTree(xc).query_pairs(2.0 * r.max() + target, output_type="ndarray")
+        pairs = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))] if len(pairs) else pairs.reshape(0, 2)
+        i, j = pairs[:, 0], pairs[:, 1]
+        d = xc[j] - xc[i]
+        dist = np.sqrt((d * d).sum(axis=1))
+        over = r[i] + r[j] + target - dist
+        hit = over > 0
+        disp = np.zeros_like(xc)
+        if hit.any():
+            push = d[hit] / dist[hit, None] * (0.5 * step * over[hit])[:, None]
+            np.add.at(disp, i[hit], -push)
+            np.add.at(disp, j[hit], push)
+        lo = (1.0 + r[:, None] + target) - xc
+        hi = xc - (N - r[:, None] - target)
+        disp += step * (np.maximum(lo, 0.0) - np.maximum(hi, 0.0))
+        if (not hit.any() or over[hit].max() <= slack) and max(lo.max(), hi.max()) <= slack:
+            return xc
+        xc += disp
+    raise RuntimeError(f"could not relax the packing to gaps of {min_gap}")
 
 
 @dataclass

@@ -2,15 +2,20 @@
 the reference's MPI/Micro_particles case4 scheme) through the C-ABI.
 
 Bars: the particle-to-node map is bit-exact (against the oracle's map and
-against brute-force enumeration of the same centres); FP64 fields within
-1e-12 relative max of the oracle (for a flow that starts at rest and stays
-slow, velocities within 1e-12 of the settling velocity scale instead); sphere
-positions, velocities and angular velocities within 1e-12 relative of the
-oracle (force and torque within 1e-10 of the largest single-link momentum
-exchange, since they are sums of cancelling terms); FP32 storage against the
-FP64 GPU path with the bound in the test.  The 3-D generalisation itself has no reference output (the
-reference's particle code is 2-D and needs MPI): parity unpinned against the
-reference, pinned against the oracle's closed-form checks."""
+against brute-force enumeration of the same centres); sphere positions,
+velocities and angular velocities within 1e-12 relative of the oracle (force
+and torque within 1e-10 of the largest single-link momentum exchange, since
+they are sums of cancelling terms); FP64 fields within FIELD_TOL = 1e-9
+relative max of the oracle.  The field bar is set by the reference's wall-
+distance bisection, not by arithmetic: calQ stops at |d - r| < 1e-9
+(particle_bounceback.F90:109-116), so centres that differ in the last bit
+(FMA contraction, the order of the momentum-exchange sums) can stop it one
+halving apart, moving q by ~2^-30; the oracle itself moves its fields by
+~5e-11 relative when a centre is perturbed by 1e-15 (tested on the CPU in
+test_oracle_particles.py).  FP32 storage is checked against the FP64 GPU path
+with the bound in the test.  The 3-D generalisation has no reference output
+(the reference's particle code is 2-D and needs MPI): parity unpinned
+against the reference, pinned against the oracle's closed-form checks."""
 import numpy as np
 import pytest
 
@@ -18,7 +23,8 @@ from mglc_amd.lbm import Cavity, CavityConfig, LbmError, random_spheres
 
 pytestmark = pytest.mark.gpu
 
-LBM_ERR_ARG, LBM_ERR_UNSUPPORTED, LBM_ERR_PARTICLE = 1, 7, 8
+LBM_ERR_ARG, LBM_ERR_STATE, LBM_ERR_UNSUPPORTED, LBM_ERR_PARTICLE = 1, 6, 7, 8
+FIELD_TOL = 1e-9
 
 
 def relmax(a, b):
@@ -91,7 +97,7 @@ def test_fp64_moving_spheres_match_oracle(oracle_lib):
     assert np.array_equal(got["owner"], ref["owner"])
     assert np.array_equal(got["owner"], brute_map(got["owner"].shape, got["xc"], r))
     for name in ("rho", "u", "v", "w"):
-        assert relmax(got[name], ref[name]) <= 1e-12, name
+        assert relmax(got[name], ref[name]) <= FIELD_TOL, name
     for name in ("xc", "U", "Om"):
         assert relmax(got[name], ref[name]) <= 1e-12, name
     link_scale = 1.0 / 18.0  # largest f in a link term
@@ -110,14 +116,13 @@ def test_fp64_settling_sphere_matches_oracle(oracle_lib, collision):
                          steps, xc, r)
     assert ref["rc"] == 0 and got["U"][0, 2] < 0
     assert np.array_equal(got["owner"], ref["owner"])
-    for name in ("rho", "xc"):
-        assert relmax(got[name], ref[name]) <= 1e-12, name
-    # The flow starts at rest and stays slow (|u| ~ 1e-5 after 150 steps), so
-    # velocities are held to the settling velocity scale Us = sqrt(g r (rho_s - 1))
-    # and the (symmetry-breaking, ~1e-7) angular velocity to Us / r.
+    assert relmax(got["xc"], ref["xc"]) <= 1e-12
+    for name in ("rho", "u", "v", "w"):
+        assert relmax(got[name], ref[name]) <= FIELD_TOL, name
+    # the sphere velocity against the settling velocity scale Us = sqrt(g r (rho_s - 1))
+    # and the (symmetry-breaking, ~1e-7) angular velocity against Us / r
     Us = float(np.sqrt(g * r[0] * 0.5))
-    for name in ("u", "v", "w", "U"):
-        assert np.abs(got[name] - ref[name]).max() <= 1e-12 * Us, name
+    assert np.abs(got["U"] - ref["U"]).max() <= 1e-12 * Us
     assert np.abs(got["Om"] - ref["Om"]).max() <= 1e-12 * Us / r[0]
 
 
@@ -158,18 +163,25 @@ def test_fp32_trajectories_against_fp64():
 
 
 def test_stop_codes():
-    # interpenetration (particle_force.F90:117-124)
+    # interpenetration (particle_force.F90:117-124), reported with its step;
+    # the context then refuses to step until the spheres are attached again
     with Cavity(box(24, 24, 24)) as c:
         c.particles_init(np.array([[8.0, 12.0, 12.0], [13.0, 12.0, 12.0]]), np.array([3.0, 3.0]))
         with pytest.raises(LbmError) as e:
+            c.step(3)
+        assert e.value.status == LBM_ERR_PARTICLE and "step 1:" in str(e.value)
+        assert "interpenetration" in str(e.value)
+        with pytest.raises(LbmError) as e:
             c.step(1)
-        assert e.value.status == LBM_ERR_PARTICLE
+        assert e.value.status == LBM_ERR_STATE
+        c.particles_init(np.array([[8.0, 12.0, 12.0], [16.0, 12.0, 12.0]]), np.array([3.0, 3.0]))
+        c.step(2)
     # wall penetration (particle_force.F90:136-186)
     with Cavity(box(24, 24, 24)) as c:
         c.particles_init(np.array([[2.5, 12.0, 12.0]]), np.array([3.0]))
         with pytest.raises(LbmError) as e:
             c.step(1)
-        assert e.value.status == LBM_ERR_PARTICLE
+        assert e.value.status == LBM_ERR_PARTICLE and "wall" in str(e.value)
 
 
 def test_bad_arguments_and_unsupported_contexts():

@@ -103,3 +103,20 @@ def test_interpenetration_and_wall_penetration_stop_the_run(oracle_lib):
     p1 = O.particle_params(24, 24, 24, 1, 0.8)
     out = O.particle_run(p1, 1, np.array([[2.5, 12.0, 12.0]]), np.array([3.0]))
     assert out["rc"] == -3
+
+
+def test_fields_are_conditioned_by_the_bisection_stop(oracle_lib):
+    """calQ stops at |d - r| < 1e-9 (particle_bounceback.F90:109-116): a
+    last-bit change of a centre can move q by ~2^-30 on some link, which
+    moves the flow by far more than rounding.  Perturbing the centre by
+    1e-15 relative changes the fields by more than 1e-13 but stays below
+    1e-9 relative: the bar the GPU parity tests use for particle fields."""
+    O = oracle_lib
+    n = (24, 22, 30)
+    xc, r = np.array([[12.3, 11.1, 17.6]]), np.array([4.7])
+    p = O.particle_params(*n, 1, 0.7, gravity=2e-4)
+    a = O.particle_run(p, 150, xc, r)
+    b = O.particle_run(p, 150, xc * (1 + 1e-15), r)
+    d = np.abs(a["u"] - b["u"]).max() / np.abs(a["u"]).max()
+    assert 1e-13 < d < 1e-9
+    assert np.abs(a["xc"] - b["xc"]).max() < 1e-12
