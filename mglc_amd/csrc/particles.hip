@@ -2,38 +2,51 @@
 // the reference couples discs to D2Q9 (MPI/Micro_particles/fortran/case4/mpi).
 //
 // Per step, all on the device (no host round trip):
-//   k_fluid_mean      rhoAvg over fluid nodes of the last macro
+//   k_fluid_sum/mean  rhoAvg over fluid nodes of the last macro
 //                     (particle_bounceback.F90:15-35)
-//   k_particle_links  one block per sphere: its boundary links compacted in
-//                     LDS; per link the wall distance q by bisection, the
-//                     quadratic interpolated bounce-back value with the
-//                     moving-wall term (particle_bounceback.F90:37-141), and
-//                     the momentum exchange force and torque
-//                     (particle_force.F90:31-92) summed in FP64 in a fixed
-//                     order (deterministic)
-//   k_particle_step   per fluid node: masked pull streaming with box-wall
-//                     bounce-back and the link values, macro, collision
-//   k_particle_update one thread per sphere: spring and buoyancy forces and
-//                     the kinematics (particle_force.F90:96-191,
+//   k_link_scan       one block per sphere: the boundary links (fluid node x,
+//                     direction a with x + e_a inside the sphere) of the
+//                     sphere's shell, compacted in a fixed order into the
+//                     sphere's slab of the link list
+//   k_link_prefix     waves of 64 links per sphere, prefix-summed; k_link_waves
+//                     tabulates the sphere of each wave
+//   k_link_proc       one lane per link over the whole list: wall distance q
+//                     by bisection, quadratic interpolated bounce-back with the
+//                     moving-wall term (particle_bounceback.F90:37-141), the
+//                     value stored in the solid node's own slot of the source
+//                     buffer (the slot the fluid node pulls from), and the
+//                     link's momentum exchange and torque
+//                     (particle_force.F90:31-92) summed per wave in FP64
+//   k_link_reduce     per sphere, the wave sums in order: force and torque are
+//                     deterministic
+//   k_particle_step   every node: plain pull streaming with halfway bounce-back
+//                     at the box walls, macro (fluid nodes), collision
+//   k_particle_update one wave per sphere: spring and buoyancy forces and the
+//                     kinematics (particle_force.F90:96-191,
 //                     particle_update.F90:26-42); stop codes for
 //                     interpenetration and wall penetration
 //   k_particle_mark   new solid map, rho_s and u = 0 in solids
 //                     (particle_update.F90:69-91)
-//   k_fluid_mean      rhoAvg over the new fluid nodes
+//   k_fluid_sum/mean  rhoAvg over the new fluid nodes
 //   k_particle_refill one block per sphere: uncovered nodes refilled by 3-point
 //                     extrapolation along the most outward lattice direction
 //                     and moment replacement with the body velocity
 //                     (particle_update.F90:110-186), then collided
+// Solid nodes are streamed and collided like fluid ones (their values are
+// never read), which keeps the fluid kernel's waves uniform.
 // The 3-D choices (volume 4/3 pi r^3, I = 2/5 m r^2, springs on all six
 // walls, rest equilibrium for stencil points that are not fluid nodes) are
 // those of the CPU restatement the GPU path is tested against.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <new>
+#include <vector>
 
 #include "collide.hpp"
 #include "context.hpp"
+#include "sphere_geom.hpp"
 #include "step.hpp"
 
 using namespace lbm;
@@ -48,7 +61,15 @@ struct ParticleState {
     double* Ftot = nullptr;  // total force (3 np)
     int* owner[2] = {nullptr, nullptr};
     int cur = 0;             // current map / sphere state
-    void* fstream = nullptr; // link values: f(abar, x) on links out of a solid (storage type, 19 ps)
+    // boundary-link list: a slab per sphere, sized for its largest shell
+    unsigned* items = nullptr;
+    long long* slab_off = nullptr;  // np: first entry of each slab
+    int* slab_cap = nullptr;        // np: entries of each slab
+    int* link_count = nullptr;      // np: links found this step
+    int* wave_off = nullptr;        // np + 1: first wave of each sphere, total
+    int* wave_sphere = nullptr;     // max_waves: the sphere of each wave
+    double* partial = nullptr;      // 6 per wave: force and torque sums
+    long long max_waves = 0;
     double* scal = nullptr;  // [0] rhoAvg of the streaming, [1] rhoAvg of the refill
     double* red = nullptr;   // reduction partials
     int* err = nullptr;      // device stop record {code, step}; code 0 = running
@@ -62,14 +83,14 @@ constexpr double PI = 3.14159265358979323846;
 struct PArgs {
     const void* src;
     void* dst;
-    void* fstream;
+    void* src_w;  // the source buffer, written by k_link_proc at solid nodes only
     int nx, ny, nz, px;
     long long pxy, ps;
     double snu, sq;
     float scale, inv_scale;
     int meq12_rho;
     double rho0, rho_solid;
-    const int* owner;      // map of this step
+    const int* owner;      // map of this step: sphere index, -1 for fluid
     const int* owner_new;  // map after the sphere update (refill)
     const double *xc, *U, *Om;           // sphere state of this step
     const double *xc_new, *U_new, *Om_new;  // after the update (refill)
@@ -79,47 +100,23 @@ struct PArgs {
     int* err;  // {code, step}: first stop wins
     int step;  // number of the step being made
     int np;
+    // boundary links
+    unsigned* items;
+    const long long* slab_off;
+    const int* slab_cap;
+    int* link_count;
+    int* wave_off;
+    int* wave_sphere;
+    double* partial;
     // sphere update
     double gravity, threshold_wall, stiff_wall, threshold_particle, stiff_particle;
     double *F, *T, *Ftot;
     double *xc_out, *U_out, *Om_out;
 };
 
-__device__ __forceinline__ double dist2(double dx, double dy, double dz) {
-    // products rounded separately and summed left to right, as the reference's
-    // x**2.0d0 sums compile at -O3: the solid map is bit-exact
-    return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-}
+using lbm::calq;
 
-// calQ (particle_bounceback.F90:101-141) for the link from node X along
-// e = (ex, ey, ez) into sphere (cx, cy, cz, rad).  Returns false when q
-// leaves [0, 1].
-__device__ __noinline__ bool calq(double cx, double cy, double cz, double rad, double X, double Y, double Z, int ex, int ey,
-                     int ez, double& x0, double& y0, double& z0, double& qout) {
-    double q = 0.5, qt = 0.5;
-    double px = X + qt * ex, py = Y + qt * ey, pz = Z + qt * ez;
-    for (int it = 0; it < 200; ++it) {
-        const double d = sqrt(dist2(px - cx, py - cy, pz - cz));
-        if (fabs(d - rad) < 1e-9) break;
-        qt = qt / 2.0;
-        if (d > rad) {
-            px += qt * ex;
-            py += qt * ey;
-            pz += qt * ez;
-            q += qt;
-        } else {
-            px -= qt * ex;
-            py -= qt * ey;
-            pz -= qt * ez;
-            q -= qt;
-        }
-    }
-    x0 = px;
-    y0 = py;
-    z0 = pz;
-    qout = q;
-    return !(q > 1.0 || q < 0.0);
-}
+__device__ __forceinline__ double dist2(double dx, double dy, double dz) { return sphere_dist2(dx, dy, dz); }
 
 __device__ __forceinline__ void body_velocity(const double* xc, const double* U, const double* Om, int id, double x,
                                               double y, double z, double& ux, double& uy, double& uz) {
@@ -171,145 +168,263 @@ __device__ __forceinline__ void sphere_box(const PArgs& a, const double* xc, int
     }
 }
 
+// Rows k of column (i, j) (0-based, clamped to [k_lo, k_hi]) that can hold
+// nodes within distance R of the centre (cx, cy, cz) (1-based coordinates):
+// the chord |z - cz| <= sqrt(R^2 - dx^2 - dy^2), widened by a node on each
+// side so rounding cannot lose one.  Callers apply the exact test per node.
+__device__ __forceinline__ void column_rows(double cx, double cy, double cz, double R, int i, int j, int k_lo, int k_hi,
+                                            int& k0, int& k1) {
+    const double dx = i + 1.0 - cx, dy = j + 1.0 - cy;
+    const double rho2 = dx * dx + dy * dy;
+    if (rho2 > (R + 1.0) * (R + 1.0)) {
+        k0 = 1;
+        k1 = 0;
+        return;
+    }
+    const double h = sqrt(fmax(R * R - rho2, 0.0));
+    k0 = (int)floor(cz - h) - 2;
+    k1 = (int)ceil(cz + h);
+    k0 = k0 < k_lo ? k_lo : k0;
+    k1 = k1 > k_hi ? k_hi : k1;
+}
+
+// A link list entry: node offsets in the sphere's bounding box (9 bits each;
+// boxes are at most 2 r + 5 <= 505 nodes wide) and the direction (5 bits).
+__device__ __forceinline__ unsigned link_entry(int di, int dj, int dk, int al) {
+    return ((unsigned)di << 23) | ((unsigned)dj << 14) | ((unsigned)dk << 5) | (unsigned)al;
+}
+
 __device__ __forceinline__ double wave_sum(double x) {
     for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
     return x;
 }
 
-constexpr int LINK_THREADS = 256;
-constexpr int LINK_CAP = 8192;                     // LDS link list (32 KiB)
-constexpr int LINK_CHUNK_MAX = LINK_THREADS * 18;  // links one chunk of nodes can add
+constexpr int SCAN_THREADS = 256;
 
-// Boundary links of one sphere: every (fluid node x, direction a) with x + e_a
-// owned by the sphere.  The block walks the sphere's bounding box in chunks of
-// 256 nodes, compacts the links into an LDS list (block prefix sum), and then
-// processes the list densely, so the bisection for q runs on full waves:
-//   - wall distance q and boundary point x0 (calQ, particle_bounceback.F90:101-141)
-//   - the streamed population f(abar, x) by quadratic interpolated bounce-back
-//     with the moving-wall term (particle_bounceback.F90:37-96), written to
-//     the streamed-population buffer for the fluid kernel to pull
-//   - the link's momentum exchange (e_a - u_w) f_post(a) - (e_abar - u_w) f(abar)
-//     and its torque (particle_force.F90:31-92), summed in FP64 per thread in
-//     list order, then over the block in a fixed tree: deterministic.
-template <class S>
-__global__ __launch_bounds__(LINK_THREADS) void k_particle_links(const PArgs a) {
-    using T = typename S::T;
-    using R = typename S::Real;
-    if (stopped(a)) return;
+// Links (fluid node x, direction a with x + e_a owned by the sphere) of one
+// sphere's shell r < |x - c| <= r + 1.5 (the map's own distance test), in a
+// fixed order -- thread, node, direction -- into its slab.  Thread t walks
+// the bounding-box nodes t, t + 256, ... four at a time (independent loads in
+// flight); two walks: count, block prefix sum, write.
+__global__ __launch_bounds__(SCAN_THREADS) void k_link_scan(const PArgs a) {
+    __shared__ int halt;  // read once, so the whole block leaves together (barriers follow)
+    __shared__ int wave_tot[SCAN_THREADS / 64];
+    if (threadIdx.x == 0) halt = stopped(a);
+    __syncthreads();
+    if (halt) return;
     const int id = blockIdx.x;
     int lo[3], hi[3];
     sphere_box(a, a.xc, id, 1, lo, hi);
-    const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
-    const long long total = (bx > 0 && by > 0 && bz > 0) ? (long long)bx * by * bz : 0;
-    const T* src = static_cast<const T*>(a.src);
-    T* fs = static_cast<T*>(a.fstream);
+    const unsigned bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
+    const unsigned total = (hi[0] >= lo[0] && hi[1] >= lo[1] && hi[2] >= lo[2]) ? bx * by * bz : 0u;
     const double cx = a.xc[3 * id], cy = a.xc[3 * id + 1], cz = a.xc[3 * id + 2], rad = a.r[id];
-    const double rho_avg = a.rho_avg[0];
-    __shared__ unsigned items[LINK_CAP];
-    __shared__ int wave_tot[LINK_THREADS / 64];
+    const double rin2 = rad * rad, rout2 = (rad + 1.5) * (rad + 1.5);
+    unsigned* list = a.items + a.slab_off[id];
+    const int cap = a.slab_cap[id];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    int count = 0;  // list length, the same in every thread
-    const long long nchunks = (total + LINK_THREADS - 1) / LINK_THREADS;
-    for (long long ch = 0; ch < nchunks; ++ch) {
-        const long long t = ch * LINK_THREADS + threadIdx.x;
-        unsigned mask = 0;
-        if (t < total) {
-            const int i = lo[0] + (int)(t % bx), j = lo[1] + (int)((t / bx) % by),
-                      k = lo[2] + (int)(t / ((long long)bx * by));
-            if (a.owner[i + (long long)a.nx * (j + (long long)a.ny * k)] < 0) {
-                sfor<Q - 1>([&](auto B) {
-                    constexpr int al = decltype(B)::value + 1;
-                    const int ti = i + EX[al], tj = j + EY[al], tk = k + EZ[al];
-                    if (in_box(a, ti, tj, tk) && a.owner[ti + (long long)a.nx * (tj + (long long)a.ny * tk)] == id)
-                        mask |= 1u << al;
-                });
+    constexpr int UNROLL = 4;
+    // link mask of box node t (0 when t is past the box or not a shell fluid node)
+    auto links_at = [&](unsigned t, int& i, int& j, int& k) -> unsigned {
+        const unsigned ti = t % bx, tj = (t / bx) % by, tk = t / (bx * by);
+        i = lo[0] + (int)ti;
+        j = lo[1] + (int)tj;
+        k = lo[2] + (int)tk;
+        if (t >= total) return 0u;
+        const double d2 = dist2(i + 1.0 - cx, j + 1.0 - cy, k + 1.0 - cz);
+        if (!(d2 > rin2 && d2 <= rout2)) return 0u;
+        if (a.owner[i + (long long)a.nx * (j + (long long)a.ny * k)] >= 0) return 0u;
+        unsigned mk = 0;
+        sfor<Q - 1>([&](auto B) {
+            constexpr int al = decltype(B)::value + 1;
+            const int ni = i + EX[al], nj = j + EY[al], nk = k + EZ[al];
+            if (in_box(a, ni, nj, nk) && a.owner[ni + (long long)a.nx * (nj + (long long)a.ny * nk)] == id)
+                mk |= 1u << al;
+        });
+        return mk;
+    };
+    int cnt = 0;
+    for (unsigned base = 0; base < total; base += SCAN_THREADS * UNROLL) {
+        unsigned mk[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            int i, j, k;
+            mk[u] = links_at(base + u * SCAN_THREADS + threadIdx.x, i, j, k);
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) cnt += __popc(mk[u]);
+    }
+    int incl = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) wave_tot[wid] = incl;
+    __syncthreads();
+    int pos = incl - cnt, all = 0;
+    for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+        if (w < wid) pos += wave_tot[w];
+        all += wave_tot[w];
+    }
+    if (all > cap) {  // cannot happen for a slab sized by the shell bound
+        if (threadIdx.x == 0) {
+            stop(a, 5);
+            a.link_count[id] = 0;
+        }
+        return;
+    }
+    for (unsigned base = 0; base < total; base += SCAN_THREADS * UNROLL) {
+        unsigned mk[UNROLL];
+        int ii[UNROLL], jj[UNROLL], kk[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) mk[u] = links_at(base + u * SCAN_THREADS + threadIdx.x, ii[u], jj[u], kk[u]);
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            unsigned m = mk[u];
+            while (m) {
+                const int al = __ffs(m) - 1;
+                m &= m - 1;
+                list[pos++] = link_entry(ii[u] - lo[0], jj[u] - lo[1], kk[u] - lo[2], al);
             }
         }
-        const int cnt = __popc(mask);
-        int incl = cnt;
+    }
+    if (threadIdx.x == 0) a.link_count[id] = all;
+}
+
+// wave_off[id] = waves (64 links each) of the spheres before id; wave_off[np]
+// = all waves.  One block.
+__global__ __launch_bounds__(1024) void k_link_prefix(const PArgs a) {
+    __shared__ int wave_tot[16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int carry = 0;
+    for (int base = 0; base < a.np; base += 1024) {
+        const int id = base + threadIdx.x;
+        const int v = id < a.np ? (a.link_count[id] + 63) / 64 : 0;
+        int incl = v;
         for (int off = 1; off < 64; off <<= 1) {
             const int y = __shfl_up(incl, off, 64);
             if (lane >= off) incl += y;
         }
         if (lane == 63) wave_tot[wid] = incl;
         __syncthreads();
-        int pos = count + incl - cnt, chunk_total = 0;
-        for (int w = 0; w < LINK_THREADS / 64; ++w) {
-            if (w < wid) pos += wave_tot[w];
-            chunk_total += wave_tot[w];
+        int before = carry, all = 0;
+        for (int w = 0; w < 16; ++w) {
+            if (w < wid) before += wave_tot[w];
+            all += wave_tot[w];
         }
-        while (mask) {
-            const int al = __ffs(mask) - 1;
-            mask &= mask - 1;
-            items[pos++] = ((unsigned)t << 5) | (unsigned)al;
-        }
-        count += chunk_total;
+        if (id < a.np) a.wave_off[id] = before + incl - v;
+        carry += all;
         __syncthreads();
-        if (count <= LINK_CAP - LINK_CHUNK_MAX && ch != nchunks - 1) continue;
-        for (int q = threadIdx.x; q < count; q += LINK_THREADS) {
-            const unsigned it = items[q];
-            const long long tt = it >> 5;
+    }
+    if (threadIdx.x == 0) a.wave_off[a.np] = carry;
+}
+
+// wave_sphere[w] = the sphere whose links wave w carries.
+__global__ __launch_bounds__(64) void k_link_waves(const PArgs a) {
+    const int id = blockIdx.x;
+    for (int w = a.wave_off[id] + threadIdx.x; w < a.wave_off[id + 1]; w += 64) a.wave_sphere[w] = id;
+}
+
+// One lane per link, 64 links of one sphere per wave, over all spheres:
+//   - wall distance q and boundary point x0 (calQ, particle_bounceback.F90:101-141)
+//   - f(abar, x) by quadratic interpolated bounce-back with the moving-wall
+//     term (particle_bounceback.F90:37-96), stored in the source buffer at the
+//     solid node x + e_a, direction abar: exactly where the fluid node's pull
+//     along abar reads
+//   - the momentum exchange (e_a - u_w) f_post(a, x) - (e_abar - u_w) f(abar, x)
+//     and its torque (particle_force.F90:31-92), summed over the wave in FP64
+//     (fixed shuffle tree) into the wave's partial.
+template <class S>
+__global__ __launch_bounds__(256) void k_link_proc(const PArgs a) {
+    using T = typename S::T;
+    using R = typename S::Real;
+    if (__builtin_amdgcn_readfirstlane(stopped(a) ? 1 : 0)) return;
+    const int lane = threadIdx.x & 63;
+    const T* src = static_cast<const T*>(a.src);
+    T* src_w = static_cast<T*>(a.src_w);
+    const int total = a.wave_off[a.np];
+    const int stride = gridDim.x * (blockDim.x / 64);
+    for (int w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); w < total; w += stride) {
+        const int id = __builtin_amdgcn_readfirstlane(a.wave_sphere[w]);
+        const int idx = (w - a.wave_off[id]) * 64 + lane;
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        if (idx < a.link_count[id]) {
+            int lo[3], hi[3];
+            sphere_box(a, a.xc, id, 1, lo, hi);
+            const unsigned it = a.items[a.slab_off[id] + idx];
             const int al = (int)(it & 31u), bd = OPP[al];
-            const int i = lo[0] + (int)(tt % bx), j = lo[1] + (int)((tt / bx) % by),
-                      k = lo[2] + (int)(tt / ((long long)bx * by));
+            const int i = lo[0] + (int)(it >> 23), j = lo[1] + (int)((it >> 14) & 511u),
+                      k = lo[2] + (int)((it >> 5) & 511u);
             const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
+            // every value either branch of the interpolation may need, loaded
+            // before the bisection so the loads overlap it
+            const double fa = abs_pop<S>(src, a.ps, c, al, a.inv_scale);
+            const double fb = abs_pop<S>(src, a.ps, c, bd, a.inv_scale);
+            const double f1 = stencil_fpost<S>(a, src, i - EX[al], j - EY[al], k - EZ[al], al);
+            const double f2 = stencil_fpost<S>(a, src, i - 2 * EX[al], j - 2 * EY[al], k - 2 * EZ[al], al);
+            const double fb1 = stencil_fpost<S>(a, src, i - EX[al], j - EY[al], k - EZ[al], bd);
+            const double cx = a.xc[3 * id], cy = a.xc[3 * id + 1], cz = a.xc[3 * id + 2];
             double x0, y0, z0, qq;
-            if (!calq(cx, cy, cz, rad, i + 1.0, j + 1.0, k + 1.0, EX[al], EY[al], EZ[al], x0, y0, z0, qq))
+            if (!calq(cx, cy, cz, a.r[id], i + 1.0, j + 1.0, k + 1.0, EX[al], EY[al], EZ[al], x0, y0, z0, qq))
                 stop(a, 3);
             double uwx, uwy, uwz;
             body_velocity(a.xc, a.U, a.Om, id, x0, y0, z0, uwx, uwy, uwz);
             const double eu = EX[bd] * uwx + EY[bd] * uwy + EZ[bd] * uwz;
-            const double fa = abs_pop<S>(src, a.ps, c, al, a.inv_scale);
+            const double rho_avg = a.rho_avg[0];
             double val;
-            if (qq < 0.5) {
-                const double f1 = stencil_fpost<S>(a, src, i - EX[al], j - EY[al], k - EZ[al], al);
-                const double f2 = stencil_fpost<S>(a, src, i - 2 * EX[al], j - 2 * EY[al], k - 2 * EZ[al], al);
+            if (qq < 0.5)
                 val = qq * (1.0 + 2.0 * qq) * fa + (1.0 - 4.0 * qq * qq) * f1 - qq * (1.0 - 2.0 * qq) * f2 +
                       6.0 * weight(al) * rho_avg * eu;
-            } else {
-                const double fb = abs_pop<S>(src, a.ps, c, bd, a.inv_scale);
-                const double fb1 = stencil_fpost<S>(a, src, i - EX[al], j - EY[al], k - EZ[al], bd);
-                val = fa / qq / (1.0 + 2.0 * qq) + fb * (2.0 * qq - 1.0) / qq -
-                      fb1 * (2.0 * qq - 1.0) / (2.0 * qq + 1.0) + 6.0 * weight(al) * rho_avg / qq / (1.0 + 2.0 * qq) * eu;
+            else {
+                const double iq = 1.0 / qq, i12 = 1.0 / (1.0 + 2.0 * qq);
+                val = fa * iq * i12 + fb * (2.0 * qq - 1.0) * iq - fb1 * (2.0 * qq - 1.0) * i12 +
+                      6.0 * weight(al) * rho_avg * iq * i12 * eu;
             }
-            S::st(fs + (long long)bd * a.ps + c, S::SHIFT ? R(val - weight(bd)) : R(val), a.scale, 0u);
+            const long long cs = c + EX[al] + (long long)EY[al] * a.px + (long long)EZ[al] * a.pxy;
+            S::st(src_w + (long long)bd * a.ps + cs, S::SHIFT ? R(val - weight(bd)) : R(val), a.scale, 0u);
             const double dFx = (EX[al] - uwx) * fa - (EX[bd] - uwx) * val;
             const double dFy = (EY[al] - uwy) * fa - (EY[bd] - uwy) * val;
             const double dFz = (EZ[al] - uwz) * fa - (EZ[bd] - uwz) * val;
             const double rx = x0 - cx, ry = y0 - cy, rz = z0 - cz;
-            acc[0] += dFx;
-            acc[1] += dFy;
-            acc[2] += dFz;
-            acc[3] += ry * dFz - rz * dFy;
-            acc[4] += rz * dFx - rx * dFz;
-            acc[5] += rx * dFy - ry * dFx;
+            acc[0] = dFx;
+            acc[1] = dFy;
+            acc[2] = dFz;
+            acc[3] = ry * dFz - rz * dFy;
+            acc[4] = rz * dFx - rx * dFz;
+            acc[5] = rx * dFy - ry * dFx;
         }
-        count = 0;
-        __syncthreads();  // the next chunk refills the list
-    }
-    __shared__ double sh[6][LINK_THREADS / 64];
-    for (int q = 0; q < 6; ++q) {
-        const double v = wave_sum(acc[q]);
-        if (lane == 0) sh[q][wid] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
+#pragma unroll
         for (int q = 0; q < 6; ++q) {
-            double v = 0.0;
-            for (int w = 0; w < LINK_THREADS / 64; ++w) v += sh[q][w];
-            if (q < 3) a.F[3 * id + q] = v;
-            else a.T[3 * id + q - 3] = v;
+            const double v = wave_sum(acc[q]);
+            if (lane == 0) a.partial[6 * (long long)w + q] = v;
         }
     }
 }
 
-// Fluid nodes: masked pull streaming (fluid.F90:75-161) with halfway bounce-
-// back at the box walls and, on links whose source is a solid node, the
-// interpolated bounce-back value left by k_particle_links; macro
-// (fluid.F90:164-184) and the next collision (fluid.F90:1-72).  Solid nodes
-// are skipped.
-template <class S, int COLL>
+// Force and torque of each sphere: its wave partials summed in order.
+__global__ __launch_bounds__(64) void k_link_reduce(const PArgs a) {
+    if (__builtin_amdgcn_readfirstlane(stopped(a) ? 1 : 0)) return;
+    const int id = blockIdx.x, lane = threadIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int w = a.wave_off[id] + lane; w < a.wave_off[id + 1]; w += 64)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[q] += a.partial[6 * (long long)w + q];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[q] = wave_sum(acc[q]);
+    if (lane == 0) {
+        for (int q = 0; q < 3; ++q) {
+            a.F[3 * id + q] = acc[q];
+            a.T[3 * id + q] = acc[3 + q];
+        }
+    }
+}
+
+// Every node: pull streaming (fluid.F90:75-161) with halfway bounce-back at
+// the box walls -- a pull out of a solid node reads the interpolated
+// bounce-back value k_link_proc stored there -- then macro (fluid.F90:164-184;
+// rho every step for rhoAvg, u, v, w on the last step of a call) on fluid
+// nodes, and the next collision (fluid.F90:1-72).  Solid nodes run the same
+// arithmetic on values nobody reads.
+template <class S, int COLL, bool MACRO>
 __global__ __launch_bounds__(256) void k_particle_step(const PArgs a) {
     using T = typename S::T;
     using R = typename S::Real;
@@ -318,10 +433,8 @@ __global__ __launch_bounds__(256) void k_particle_step(const PArgs a) {
     const int k = blockIdx.z;
     if (i >= a.nx || j >= a.ny || stopped(a)) return;
     const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
-    if (a.owner[m] >= 0) return;
     const int c = i + a.px * (j + a.ny * k);
     const T* __restrict__ src = static_cast<const T*>(a.src);
-    const T* __restrict__ fs = static_cast<const T*>(a.fstream);
     R f[Q];
     sfor<Q>([&](auto B) {
         constexpr int bd = decltype(B)::value;
@@ -329,22 +442,24 @@ __global__ __launch_bounds__(256) void k_particle_step(const PArgs a) {
         if constexpr (bd == 0) {
             f[0] = S::ld(src + c, a.inv_scale);
         } else {
-            const int si = i - ex, sj = j - ey, sk = k - ez;
-            if (!in_box(a, si, sj, sk))
-                f[bd] = S::ld(src + (long long)OPP[bd] * a.ps + c, a.inv_scale);
-            else if (a.owner[si + (long long)a.nx * (sj + (long long)a.ny * sk)] >= 0)
-                f[bd] = S::ld(fs + (long long)bd * a.ps + c, a.inv_scale);
-            else
-                f[bd] = S::ld(src + (long long)bd * a.ps +
-                                  (c - ((long long)ex + (long long)ey * a.px + (long long)ez * a.pxy)),
-                              a.inv_scale);
+            bool in = true;
+            if constexpr (ex != 0) in = in && (unsigned)(i - ex) < (unsigned)a.nx;
+            if constexpr (ey != 0) in = in && (unsigned)(j - ey) < (unsigned)a.ny;
+            if constexpr (ez != 0) in = in && (unsigned)(k - ez) < (unsigned)a.nz;
+            const long long off = (long long)ex + (long long)ey * a.px + (long long)ez * a.pxy;
+            f[bd] = in ? S::ld(src + (long long)bd * a.ps + (c - off), a.inv_scale)
+                       : S::ld(src + (long long)OPP[bd] * a.ps + c, a.inv_scale);
         }
     });
     const Moments<R, S::SHIFT> mo = moments<R, S::SHIFT>(f);
-    a.rho[m] = S::SHIFT ? 1.0 + double(mo.drho) : double(mo.rho);
-    a.u[m] = double(mo.u);
-    a.v[m] = double(mo.v);
-    a.w[m] = double(mo.w);
+    if (a.owner[m] < 0) {
+        a.rho[m] = S::SHIFT ? 1.0 + double(mo.drho) : double(mo.rho);
+        if constexpr (MACRO) {
+            a.u[m] = double(mo.u);
+            a.v[m] = double(mo.v);
+            a.w[m] = double(mo.w);
+        }
+    }
     collide<S, COLL>(f, mo, a);
     T* __restrict__ dst = static_cast<T*>(a.dst);
     sfor<Q>([&](auto B) {
@@ -355,31 +470,37 @@ __global__ __launch_bounds__(256) void k_particle_step(const PArgs a) {
 
 // Streamed population bd of fluid node (i,j,k), as k_particle_step pulls it.
 template <class S>
-__device__ double streamed_pop(const PArgs& a, const typename S::T* src, const typename S::T* fs, int i, int j, int k,
-                               int bd) {
+__device__ double streamed_pop(const PArgs& a, const typename S::T* src, int i, int j, int k, int bd) {
     const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
     if (bd == 0) return abs_pop<S>(src, a.ps, c, 0, a.inv_scale);
     const int si = i - EX[bd], sj = j - EY[bd], sk = k - EZ[bd];
     if (!in_box(a, si, sj, sk)) return abs_pop<S>(src, a.ps, c, OPP[bd], a.inv_scale);
-    if (a.owner[si + (long long)a.nx * (sj + (long long)a.ny * sk)] >= 0) return abs_pop<S>(fs, a.ps, c, bd, a.inv_scale);
     return abs_pop<S>(src, a.ps, si + (long long)a.px * (sj + (long long)a.ny * sk), bd, a.inv_scale);
 }
 
+// One wave per sphere: the lanes split the pair loop of the particle springs,
+// the wave sums their parts (fixed shuffle tree), lane 0 adds the wall
+// springs and buoyancy and advances the sphere.
 __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.np || stopped(a)) return;
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    // p and the stop flag are uniform over the wave (the flag read by lane 0),
+    // so the whole wave takes part in the shuffles below
+    if (p >= a.np || __builtin_amdgcn_readfirstlane(stopped(a) ? 1 : 0)) return;
     const double rho_avg = a.rho_avg[0];
     const double rp = a.r[p];
     const double V = 4.0 / 3.0 * PI * rp * rp * rp;
     double fs[3] = {0.0, 0.0, 0.0};
     const double scale_p = V * (a.rho_solid - rho_avg) * a.gravity / a.stiff_particle;
-    for (int q = 0; q < a.np; ++q) {
+    const double px = a.xc[3 * p], py = a.xc[3 * p + 1], pz = a.xc[3 * p + 2];
+    for (int q = lane; q < a.np; q += 64) {
         if (q == p) continue;
-        const double dx = a.xc[3 * p] - a.xc[3 * q], dy = a.xc[3 * p + 1] - a.xc[3 * q + 1],
-                     dz = a.xc[3 * p + 2] - a.xc[3 * q + 2];
-        const double dij = sqrt(dist2(dx, dy, dz));
+        const double dx = px - a.xc[3 * q], dy = py - a.xc[3 * q + 1], dz = pz - a.xc[3 * q + 2];
         const double rs = rp + a.r[q];
-        if (dij >= rs + a.threshold_particle) continue;
+        const double reach = rs + a.threshold_particle;
+        if (fabs(dx) >= reach || fabs(dy) >= reach || fabs(dz) >= reach) continue;  // dij >= reach too
+        const double dij = sqrt(dist2(dx, dy, dz));
+        if (dij >= reach) continue;
         if (dij < rs) {
             stop(a, 1);
             continue;
@@ -390,6 +511,8 @@ __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
         fs[1] += mag * dy / dij;
         fs[2] += mag * dz / dij;
     }
+    for (int d = 0; d < 3; ++d) fs[d] = wave_sum(fs[d]);
+    if (lane != 0) return;
     const double scale_w = V * (a.rho_solid - a.rho0) * a.gravity / a.stiff_wall;
     const double N[3] = {(double)a.nx, (double)a.ny, (double)a.nz};
     for (int d = 0; d < 3; ++d) {
@@ -419,40 +542,42 @@ __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
 }
 
 // New map from the updated spheres; solids at rest with rho_s.
-__global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner_out) {
-    if (stopped(a)) return;
-    const int id = blockIdx.x;
+// Nodes of sphere id (d^2 <= r^2, initial.F90:78-89) walked column by column
+// of its bounding box; f(i, j, k, m) is called for each.
+template <class F>
+__device__ __forceinline__ void for_sphere_nodes(const PArgs& a, const double* xc, int id, F&& f) {
     int lo[3], hi[3];
-    sphere_box(a, a.xc_new, id, 1, lo, hi);
-    const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
-    const long long total = (bx > 0 && by > 0 && bz > 0) ? (long long)bx * by * bz : 0;
-    const double cx = a.xc_new[3 * id], cy = a.xc_new[3 * id + 1], cz = a.xc_new[3 * id + 2], rr = a.r[id] * a.r[id];
-    for (long long t = threadIdx.x; t < total; t += blockDim.x) {
-        const int i = lo[0] + (int)(t % bx), j = lo[1] + (int)((t / bx) % by), k = lo[2] + (int)(t / ((long long)bx * by));
-        if (dist2(i + 1.0 - cx, j + 1.0 - cy, k + 1.0 - cz) <= rr) {
-            const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
-            atomicMax(owner_out + m, id);  // a node claimed twice goes to the higher index
-            a.rho[m] = a.rho_solid;
-            a.u[m] = 0.0;
-            a.v[m] = 0.0;
-            a.w[m] = 0.0;
-        }
+    sphere_box(a, xc, id, 1, lo, hi);
+    const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1;
+    const int ncol = (bx > 0 && by > 0 && hi[2] >= lo[2]) ? bx * by : 0;
+    const double cx = xc[3 * id], cy = xc[3 * id + 1], cz = xc[3 * id + 2], rr = a.r[id] * a.r[id];
+    for (int col = threadIdx.x; col < ncol; col += blockDim.x) {
+        const int i = lo[0] + col % bx, j = lo[1] + col / bx;
+        int k0, k1;
+        column_rows(cx, cy, cz, a.r[id], i, j, lo[2], hi[2], k0, k1);
+        for (int k = k0; k <= k1; ++k)
+            if (dist2(i + 1.0 - cx, j + 1.0 - cy, k + 1.0 - cz) <= rr)
+                f(i, j, k, i + (long long)a.nx * (j + (long long)a.ny * k));
     }
 }
 
-// Initial map (from the current state) and the fluid equilibrium at rest.
+// New map from the updated spheres; solids at rest with rho_s.
+__global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner_out) {
+    if (stopped(a)) return;
+    const int id = blockIdx.x;
+    for_sphere_nodes(a, a.xc_new, id, [&](int, int, int, long long m) {
+        atomicMax(owner_out + m, id);  // a node claimed twice goes to the higher index
+        a.rho[m] = a.rho_solid;
+        a.u[m] = 0.0;
+        a.v[m] = 0.0;
+        a.w[m] = 0.0;
+    });
+}
+
+// Initial map (from the current state).
 __global__ __launch_bounds__(256) void k_particle_init_mark(const PArgs a, int* owner_out) {
     const int id = blockIdx.x;
-    int lo[3], hi[3];
-    sphere_box(a, a.xc, id, 1, lo, hi);
-    const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
-    const long long total = (bx > 0 && by > 0 && bz > 0) ? (long long)bx * by * bz : 0;
-    const double cx = a.xc[3 * id], cy = a.xc[3 * id + 1], cz = a.xc[3 * id + 2], rr = a.r[id] * a.r[id];
-    for (long long t = threadIdx.x; t < total; t += blockDim.x) {
-        const int i = lo[0] + (int)(t % bx), j = lo[1] + (int)((t / bx) % by), k = lo[2] + (int)(t / ((long long)bx * by));
-        if (dist2(i + 1.0 - cx, j + 1.0 - cy, k + 1.0 - cz) <= rr)
-            atomicMax(owner_out + i + (long long)a.nx * (j + (long long)a.ny * k), id);
-    }
+    for_sphere_nodes(a, a.xc, id, [&](int, int, int, long long m) { atomicMax(owner_out + m, id); });
 }
 
 // rho = rho_s in solids and rho0 in the fluid, u = 0, and the first
@@ -497,17 +622,11 @@ __global__ __launch_bounds__(256) void k_particle_refill(const PArgs a) {
     using R = typename S::Real;
     if (stopped(a)) return;
     const int id = blockIdx.x;
-    int lo[3], hi[3];
-    sphere_box(a, a.xc, id, 1, lo, hi);  // old position
-    const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
-    const long long total = (bx > 0 && by > 0 && bz > 0) ? (long long)bx * by * bz : 0;
     const T* src = static_cast<const T*>(a.src);
-    const T* fs = static_cast<const T*>(a.fstream);
     const double rho_avg = a.rho_avg[1];
-    for (long long t = threadIdx.x; t < total; t += blockDim.x) {
-        const int i = lo[0] + (int)(t % bx), j = lo[1] + (int)((t / bx) % by), k = lo[2] + (int)(t / ((long long)bx * by));
-        const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
-        if (!(a.owner[m] == id && a.owner_new[m] < 0)) continue;
+    // uncovered nodes: inside the sphere's old position, fluid in the new map
+    for_sphere_nodes(a, a.xc, id, [&](int i, int j, int k, long long m) {
+        if (!(a.owner[m] == id && a.owner_new[m] < 0)) return;
         // extrapolation direction: most outward lattice direction w.r.t. the new centre
         const double rx = i + 1.0 - a.xc_new[3 * id], ry = j + 1.0 - a.xc_new[3 * id + 1],
                      rz = k + 1.0 - a.xc_new[3 * id + 2];
@@ -523,7 +642,7 @@ __global__ __launch_bounds__(256) void k_particle_refill(const PArgs a) {
         }
         if (ec == 0) {
             stop(a, 4);
-            continue;
+            return;
         }
         double fr[Q];
         for (int al = 0; al < Q; ++al) fr[al] = 0.0;
@@ -536,7 +655,7 @@ __global__ __launch_bounds__(256) void k_particle_refill(const PArgs a) {
             }
             const double coef = s == 1 ? 3.0 : (s == 2 ? -3.0 : 1.0);
             for (int al = 0; al < Q; ++al) {
-                const double v = ok ? streamed_pop<S>(a, src, fs, pi, pj, pk, al) : weight(al) * a.rho0;
+                const double v = ok ? streamed_pop<S>(a, src, pi, pj, pk, al) : weight(al) * a.rho0;
                 fr[al] = s == 1 ? coef * v : fr[al] + coef * v;
             }
         }
@@ -560,7 +679,7 @@ __global__ __launch_bounds__(256) void k_particle_refill(const PArgs a) {
         const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
         T* __restrict__ dst = static_cast<T*>(a.dst);
         for (int al = 0; al < Q; ++al) S::st(dst + (long long)al * a.ps + c, f[al], a.scale, 0u);
-    }
+    });
 }
 
 // Sum of rho and count over fluid nodes, then rhoAvg into out.
@@ -663,7 +782,13 @@ PArgs particle_args(const lbm_ctx* c) {
     a.F = s->F;
     a.T = s->T;
     a.Ftot = s->Ftot;
-    a.fstream = s->fstream;
+    a.items = s->items;
+    a.slab_off = s->slab_off;
+    a.slab_cap = s->slab_cap;
+    a.link_count = s->link_count;
+    a.wave_off = s->wave_off;
+    a.wave_sphere = s->wave_sphere;
+    a.partial = s->partial;
     return a;
 }
 
@@ -678,17 +803,26 @@ void launch_fluid_mean(lbm_ctx* c, const int* owner, double* out) {
 }
 
 template <class S, int COLL>
-void particle_launches(lbm_ctx* c, int step) {
+void particle_launches(lbm_ctx* c, int step, bool macro) {
     ParticleState* s = c->part;
     const int np = s->np;
     launch_fluid_mean(c, s->owner[s->cur], s->scal);
     PArgs a = particle_args(c);
     a.step = step;
     a.src = c->buf[c->cur];
+    a.src_w = c->buf[c->cur];
     a.dst = c->buf[1 - c->cur];
-    hipLaunchKernelGGL(k_particle_links<S>, dim3(np), dim3(LINK_THREADS), 0, c->stream, a);
-    hipLaunchKernelGGL((k_particle_step<S, COLL>), cell_grid(c), c->block, 0, c->stream, a);
-    hipLaunchKernelGGL(k_particle_update, dim3((np + 255) / 256), dim3(256), 0, c->stream, a);
+    const int proc_blocks = (int)std::min<long long>(2048, (s->max_waves + 3) / 4);
+    hipLaunchKernelGGL(k_link_scan, dim3(np), dim3(SCAN_THREADS), 0, c->stream, a);
+    hipLaunchKernelGGL(k_link_prefix, dim3(1), dim3(1024), 0, c->stream, a);
+    hipLaunchKernelGGL(k_link_waves, dim3(np), dim3(64), 0, c->stream, a);
+    hipLaunchKernelGGL(k_link_proc<S>, dim3(proc_blocks), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_link_reduce, dim3(np), dim3(64), 0, c->stream, a);
+    if (macro)
+        hipLaunchKernelGGL((k_particle_step<S, COLL, true>), cell_grid(c), c->block, 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_particle_step<S, COLL, false>), cell_grid(c), c->block, 0, c->stream, a);
+    hipLaunchKernelGGL(k_particle_update, dim3((np + 3) / 4), dim3(256), 0, c->stream, a);
     hipLaunchKernelGGL(k_fill_int, dim3(2048), dim3(256), 0, c->stream, s->owner[1 - s->cur], c->n, -1);
     hipLaunchKernelGGL(k_particle_mark, dim3(np), dim3(256), 0, c->stream, a, s->owner[1 - s->cur]);
     launch_fluid_mean(c, s->owner[1 - s->cur], s->scal + 1);
@@ -704,8 +838,9 @@ void free_state(ParticleState* s) {
         if (s->Om[q]) (void)hipFree(s->Om[q]);
         if (s->owner[q]) (void)hipFree(s->owner[q]);
     }
-    for (void* p : {(void*)s->r, (void*)s->F, (void*)s->T, (void*)s->Ftot, s->fstream, (void*)s->scal, (void*)s->red,
-                    (void*)s->err})
+    for (void* p : {(void*)s->r, (void*)s->F, (void*)s->T, (void*)s->Ftot, (void*)s->items, (void*)s->slab_off,
+                    (void*)s->slab_cap, (void*)s->link_count, (void*)s->wave_off, (void*)s->wave_sphere, (void*)s->partial, (void*)s->scal,
+                    (void*)s->red, (void*)s->err})
         if (p) (void)hipFree(p);
 }
 
@@ -724,12 +859,13 @@ int particles_step(lbm_ctx* c, int nsteps) {
     const bool bgk = c->p.collision == LBM_BGK;
     for (int q = 0; q < nsteps; ++q) {
         const int step = (int)(c->itc + q + 1);
+        const bool macro = q == nsteps - 1;
         if (c->p.storage == LBM_STORE_F32) {
-            if (bgk) particle_launches<StoreF32, COLL_BGK>(c, step);
-            else particle_launches<StoreF32, COLL_MRT>(c, step);
+            if (bgk) particle_launches<StoreF32, COLL_BGK>(c, step, macro);
+            else particle_launches<StoreF32, COLL_MRT>(c, step, macro);
         } else {
-            if (bgk) particle_launches<StoreF64, COLL_BGK>(c, step);
-            else particle_launches<StoreF64, COLL_MRT>(c, step);
+            if (bgk) particle_launches<StoreF64, COLL_BGK>(c, step, macro);
+            else particle_launches<StoreF64, COLL_MRT>(c, step, macro);
         }
     }
     int rc = lbm_check_launch();
@@ -743,9 +879,10 @@ int particles_step(lbm_ctx* c, int nsteps) {
         "", "particle-particle interpenetration (particle_force.F90:117-124)",
         "a particle penetrates a wall (particle_force.F90:136-186)",
         "wall distance q outside [0, 1] (particle_bounceback.F90:128-138)",
-        "no extrapolation direction for an uncovered node (non-finite sphere state)"};
+        "no extrapolation direction for an uncovered node (non-finite sphere state)",
+        "boundary-link list of a sphere overflowed"};
     return lbm_fail(LBM_ERR_PARTICLE, "particle run stopped at step %d: %s", err[1],
-                    err[0] >= 1 && err[0] <= 4 ? why[err[0]] : "unknown stop code");
+                    err[0] >= 1 && err[0] <= 5 ? why[err[0]] : "unknown stop code");
 }
 
 extern "C" {
@@ -787,7 +924,32 @@ int lbm_particles_init(lbm_ctx* c, const lbm_particle_params* pp, const double* 
     if (e == hipSuccess) e = hipMalloc(&s->F, v3);
     if (e == hipSuccess) e = hipMalloc(&s->T, v3);
     if (e == hipSuccess) e = hipMalloc(&s->Ftot, v3);
-    if (e == hipSuccess) e = hipMalloc(&s->fstream, c->elem * (size_t)c->ps * Q);
+    // link slabs: a sphere of radius r has at most 18 links per node of its
+    // shell r < d <= r + 1.5, and those nodes' unit cells lie inside the
+    // shell r - 0.87 < d <= r + 2.37, so its volume bounds their number
+    std::vector<long long> off(pp->np);
+    std::vector<int> cap(pp->np);
+    long long total_cap = 0;
+    s->max_waves = 0;
+    for (int q = 0; q < pp->np; ++q) {
+        const double ro = radii[q] + 2.4, ri = radii[q] > 0.9 ? radii[q] - 0.9 : 0.0;
+        const double nodes = 4.0 / 3.0 * 3.14159265358979323846 * (ro * ro * ro - ri * ri * ri);
+        const long long c18 = 18 * ((long long)nodes + 1);
+        cap[q] = (int)((c18 + 63) / 64 * 64);
+        off[q] = total_cap;
+        total_cap += cap[q];
+        s->max_waves += cap[q] / 64;
+    }
+    if (e == hipSuccess) e = hipMalloc(&s->items, sizeof(unsigned) * (size_t)total_cap);
+    if (e == hipSuccess) e = hipMalloc(&s->slab_off, sizeof(long long) * pp->np);
+    if (e == hipSuccess) e = hipMalloc(&s->slab_cap, sizeof(int) * pp->np);
+    if (e == hipSuccess) e = hipMalloc(&s->link_count, sizeof(int) * pp->np);
+    if (e == hipSuccess) e = hipMalloc(&s->wave_off, sizeof(int) * (pp->np + 1));
+    if (e == hipSuccess) e = hipMalloc(&s->partial, sizeof(double) * 6 * (size_t)s->max_waves);
+    if (e == hipSuccess) e = hipMalloc(&s->wave_sphere, sizeof(int) * (size_t)s->max_waves);
+    if (e == hipSuccess) e = hipMemcpy(s->slab_off, off.data(), sizeof(long long) * pp->np, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->slab_cap, cap.data(), sizeof(int) * pp->np, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(s->link_count, 0, sizeof(int) * pp->np);
     if (e == hipSuccess) e = hipMalloc(&s->scal, sizeof(double) * 2);
     if (e == hipSuccess) e = hipMalloc(&s->red, sizeof(double) * 2 * RED_BLOCKS);
     if (e == hipSuccess) e = hipMalloc(&s->err, 2 * sizeof(int));

@@ -1,0 +1,68 @@
+// sphere_geom.hpp -- node/sphere geometry of the particle coupling, shared by
+// the device kernels and host-side checks (host and device builds give the
+// same bits: no contraction, correctly rounded sqrt).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace lbm {
+
+// Squared distance with each product rounded and the sum taken left to
+// right, as (x - xc)**2 + (y - yc)**2 + (z - zc)**2 of initial.F90:78-89 and
+// particle_update.F90:69-79 evaluates: the solid map built from it is exact.
+__host__ __device__ inline double sphere_dist2(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+// calQ (particle_bounceback.F90:101-141): wall fraction q of the link from
+// node X along e = (ex, ey, ez) into the sphere (cx, cy, cz, rad) by
+// bisection from q = 1/2, stopping when |d - rad| < 1e-9 (at most 200
+// halvings); (x0, y0, z0) receives the last point.  Returns false when q
+// leaves [0, 1].
+// Same halvings, same stop rule and same branches as the reference's loop
+// (bit for bit), at a fraction of its cost: along the link
+// g(q) = |X + q e - c|^2 - rad^2 = (a q + 2 b) q + g0 is a quadratic, so each
+// halving costs two fused multiply-adds.  Where |g| > T the point is more
+// than 2e-9 away from the surface (|d - rad| = |g| / (d + rad), d <= rad + 3
+// on a boundary link): the reference's stop test cannot fire and its branch
+// d > rad is g > 0.  T also covers the rounding of g (below 1e-14 (rad + 2)^2,
+// allowed 1e-12 (rad + 2)^2).  Only inside that band is the point formed and
+// d computed exactly as the reference does, from the point accumulated step
+// by step as the reference accumulates it.
+__host__ __device__ inline bool calq(double cx, double cy, double cz, double rad, double X, double Y, double Z, int ex,
+                                     int ey, int ez, double& x0, double& y0, double& z0, double& qout) {
+#pragma clang fp contract(off)
+    const double Dx = X - cx, Dy = Y - cy, Dz = Z - cz;
+    const double ga = (double)(ex * ex + ey * ey + ez * ez);
+    const double gb2 = 2.0 * (ex * Dx + ey * Dy + ez * Dz);
+    const double g0 = sphere_dist2(Dx, Dy, Dz) - rad * rad;
+    const double band = (2.0 * rad + 3.0) * 2e-9 + 1e-12 * (rad + 2.0) * (rad + 2.0);
+    double q = 0.5, qt = 0.5;
+    double px = X + qt * ex, py = Y + qt * ey, pz = Z + qt * ez;
+    for (int it = 0; it < 200; ++it) {
+        const double g = __builtin_fma(__builtin_fma(ga, q, gb2), q, g0);
+        bool outside;
+        if (fabs(g) <= band) {
+            const double d = sqrt(sphere_dist2(px - cx, py - cy, pz - cz));
+            if (fabs(d - rad) < 1e-9) break;
+            outside = d > rad;
+        } else {
+            outside = g > 0.0;
+        }
+        qt = qt / 2.0;
+        const double s = outside ? qt : -qt;
+        px += s * ex;
+        py += s * ey;
+        pz += s * ez;
+        q += s;
+    }
+    x0 = px;
+    y0 = py;
+    z0 = pz;
+    qout = q;
+    return !(q > 1.0 || q < 0.0);
+}
+
+}  // namespace lbm
