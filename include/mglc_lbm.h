@@ -193,6 +193,22 @@ int lbm_synchronize(lbm_ctx *ctx);
 
 int lbm_get_info(const lbm_ctx *ctx, lbm_info *info);
 
+/* ---- checkpoint / restart ---------------------------------------------------
+ * Replaces backupData (Buoyancy_driven_cavity/fortran/3d/seq/bouyancy3d.F90:
+ * 1011-1030) and the loadInitField restart switch
+ * (3d/mpi_blocked/initial.F90:102, 195-206).  The state blob holds what the
+ * next step reads (stored populations in the device layout, lid density,
+ * macro fields, residual history, incoming halo, thermal populations, force
+ * and temperature) behind a header naming the grid, storage, collision and
+ * flags it belongs to; a run restarted from it continues bit for bit.  The
+ * caller owns the host buffer and its file I/O.  Contexts with attached
+ * spheres return LBM_ERR_UNSUPPORTED. */
+int lbm_state_bytes(lbm_ctx *ctx, size_t *bytes);
+int lbm_save_state(lbm_ctx *ctx, void *host_blob, size_t bytes);
+/* The context must have the blob's grid, storage, collision, thermal and
+ * halo settings (else LBM_ERR_ARG); the step counter is restored. */
+int lbm_load_state(lbm_ctx *ctx, const void *host_blob, size_t bytes);
+
 /* ---- moving spheres --------------------------------------------------------
  * Rigid spheres in a closed box (no-slip walls, no lid), coupled to the
  * D3Q19 fluid as the reference couples discs to D2Q9

@@ -29,13 +29,15 @@ module commondata
     logical :: write_centerlines = .true.
     logical :: write_binary = .false.
     logical :: write_raw = .false.                ! rho,u,v,w as raw FP64 stream (x fastest)
+    character(len=256) :: restart_file = ''       ! start from this saved state instead of initial
+    character(len=256) :: backup_file = ''        ! save the final state here (exact restart)
 
     real(8), allocatable :: xp(:), yp(:), zp(:)
     real(8), allocatable :: u(:, :, :), v(:, :, :), w(:, :, :), rho(:, :, :)
 
     namelist /cavity/ total_nx, total_ny, total_nz, Reynolds, rho0, U0, itc_max, eps, check_every, &
                       collision_model, storage, block_x, block_y, halo_loopback, output_dir, write_tecplot, &
-                      write_centerlines, write_binary, write_raw
+                      write_centerlines, write_binary, write_raw, restart_file, backup_file
 
 contains
 

@@ -5,6 +5,8 @@
 !   lbm_step(n)        n x (collision.f90 + streaming.f90 + bounce_back.f90 + macro.f90)
 !   lbm_check          check.f90:9-34 (returns errorU)
 !   lbm_download_macro the gather half of output.f90:1-133 (one GPU = one rank)
+!   lbm_save_state / lbm_load_state   backupData / loadInitField of the
+!                      buoyancy codes (seq/bouyancy3d.F90:1011-1030)
 module mglc_lbm_capi
     use iso_c_binding
     implicit none
@@ -34,6 +36,13 @@ module mglc_lbm_capi
         real(c_double) :: t_hot, t_cold, t_ref
         real(c_double) :: reserved_d(2)
     end type lbm_params
+
+    ! mirrors struct lbm_info
+    type, bind(C) :: lbm_info
+        integer(c_long_long) :: nx, ny, nz, pitch_x, pop_stride, steps_done, device_bytes
+        real(c_double) :: bytes_per_lup
+        integer(c_int) :: storage, collision
+    end type lbm_info
 
     interface
         integer(c_int) function lbm_create(p, ctx) bind(C, name='lbm_create')
@@ -98,6 +107,32 @@ module mglc_lbm_capi
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
         end function lbm_synchronize
+
+        integer(c_int) function lbm_state_bytes(ctx, bytes) bind(C, name='lbm_state_bytes')
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), value :: ctx
+            integer(c_size_t), intent(out) :: bytes
+        end function lbm_state_bytes
+
+        integer(c_int) function lbm_save_state(ctx, blob, bytes) bind(C, name='lbm_save_state')
+            import :: c_int, c_ptr, c_size_t, c_int8_t
+            type(c_ptr), value :: ctx
+            integer(c_int8_t), intent(out) :: blob(*)
+            integer(c_size_t), value :: bytes
+        end function lbm_save_state
+
+        integer(c_int) function lbm_load_state(ctx, blob, bytes) bind(C, name='lbm_load_state')
+            import :: c_int, c_ptr, c_size_t, c_int8_t
+            type(c_ptr), value :: ctx
+            integer(c_int8_t), intent(in) :: blob(*)
+            integer(c_size_t), value :: bytes
+        end function lbm_load_state
+
+        integer(c_int) function lbm_get_info(ctx, info) bind(C, name='lbm_get_info')
+            import :: c_int, c_ptr, lbm_info
+            type(c_ptr), value :: ctx
+            type(lbm_info), intent(out) :: info
+        end function lbm_get_info
 
         type(c_ptr) function lbm_last_error_c() bind(C, name='lbm_last_error')
             import :: c_ptr

@@ -35,6 +35,7 @@ EXPORTED_SYMBOLS = (
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_initial", "lbm_batch_step", "lbm_batch_step_each",
     "lbm_batch_autotune", "lbm_batch_set_launch", "lbm_batch_synchronize", "lbm_batch_download_macro",
     "lbm_batch_get_info", "lbm_particles_init", "lbm_particles_get", "lbm_particles_map",
+    "lbm_state_bytes", "lbm_save_state", "lbm_load_state",
 )
 
 FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = range(6)
@@ -133,6 +134,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lbm_particles_init.argtypes = [vp, ctypes.POINTER(LbmParticleParams), dp, dp, dp, dp]
     lib.lbm_particles_get.argtypes = [vp, dp, dp, dp, dp, dp]
     lib.lbm_particles_map.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    lib.lbm_state_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+    lib.lbm_save_state.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.lbm_load_state.argtypes = [vp, vp, ctypes.c_size_t]
     lib.lbm_status_string.argtypes = [ip]
     lib.lbm_status_string.restype = ctypes.c_char_p
     lib.lbm_last_error.restype = ctypes.c_char_p
@@ -426,6 +430,18 @@ class Cavity:
 
     def synchronize(self):
         self._call("lbm_synchronize", self._ctx)
+
+    def save_state(self) -> bytes:
+        """The exact-restart state (lbm_save_state) as bytes."""
+        n = ctypes.c_size_t()
+        self._call("lbm_state_bytes", self._ctx, ctypes.byref(n))
+        buf = ctypes.create_string_buffer(n.value)
+        self._call("lbm_save_state", self._ctx, buf, n)
+        return buf.raw
+
+    def load_state(self, blob: bytes):
+        buf = ctypes.create_string_buffer(bytes(blob), len(blob))
+        self._call("lbm_load_state", self._ctx, buf, ctypes.c_size_t(len(blob)))
 
     def particles_init(self, centers, radii, rho_solid=1.5, gravity=1e-5, threshold_wall=4.0, stiff_wall=0.10,
                        threshold_particle=4.0, stiff_particle=0.10, U=None, omega=None):

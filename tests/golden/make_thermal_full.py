@@ -3,7 +3,7 @@
 The reference's buoyancy parameters are compile-time (module.F90: Ra = 1e6,
 N = 51), so it cannot run the 128^3, Ra = 1e5 case; the oracle -- pinned to
 the reference by the thermal_*.npz fixtures and closed forms -- runs it once
-here (about 10 minutes on 8 cores) and the summary is committed:
+here (about 80 minutes on 8 cores) and the summary is committed:
 Nusselt numbers, residuals, mid-plane profiles and a strided subsample of
 the fields after 20000 steps.
 

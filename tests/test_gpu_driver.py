@@ -49,3 +49,37 @@ def test_fortran_driver_matches_oracle(tmp_path, oracle_lib):
     uz = np.loadtxt(tmp_path / f"u-z_{steps:09d}.dat")
     assert uz.shape == (nz, 2)
     np.testing.assert_allclose(uz[:, 0], u[:, (ny - 1) // 2, (nx - 1) // 2] / 0.1, rtol=1e-10, atol=1e-13)
+
+
+def test_fortran_driver_restart_continues_bit_for_bit(tmp_path):
+    """backup_file / restart_file (the reference's backupData / loadInitField):
+    400 steps, save, restart to 800 equals one straight run of 800 steps."""
+    exe = os.path.join(B.BIN_DIR, "mglc_cavity")
+    nx, ny, nz = 20, 18, 16
+    common = (f" total_nx={nx}, total_ny={ny}, total_nz={nz}, Reynolds=80.0, U0=0.1, eps=0.0,"
+              f" check_every=200, storage='F32', write_tecplot=.false., write_centerlines=.false.,"
+              f" write_raw=.true.,")
+    runs = {
+        "straight": f"itc_max=799, output_dir='{tmp_path}/straight'",
+        "first": f"itc_max=399, output_dir='{tmp_path}/first', backup_file='{tmp_path}/state.bin'",
+        "second": f"itc_max=799, output_dir='{tmp_path}/second', restart_file='{tmp_path}/state.bin'",
+    }
+    out = {}
+    for name, extra in runs.items():
+        os.makedirs(tmp_path / name, exist_ok=True)
+        nml = tmp_path / f"{name}.nml"
+        nml.write_text(f"&cavity\n{common}\n {extra} /\n")
+        print(f"[gpu] mglc_cavity restart test: {name}", flush=True)
+        r = subprocess.run([exe, str(nml)], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+        out[name] = r.stdout
+    a = np.fromfile(tmp_path / "straight" / "cavity-raw-000000800.bin", dtype=np.float64)
+    b = np.fromfile(tmp_path / "second" / "cavity-raw-000000800.bin", dtype=np.float64)
+    assert a.size == 4 * nx * ny * nz and np.array_equal(a, b)
+    assert "restarted from" in out["second"]
+
+    def residuals(text):
+        return {int(p[0]): p[1] for p in (l.split() for l in text.splitlines()) if len(p) == 2 and p[0].isdigit()}
+
+    ra, rb = residuals(out["straight"]), residuals(out["second"])
+    assert ra[600] == rb[600] and ra[800] == rb[800]

@@ -126,3 +126,38 @@ def test_side_heated_cavity_is_centro_symmetric():
 def test_thermal_mass_conserved():
     rho, *_ = gpu_thermal(c2_like(20, 24, 22), 1500)
     assert abs(rho.sum() - rho.size) / rho.size < 1e-13
+
+
+def test_full_size_buoyancy_cavity_against_oracle_fixture():
+    """The buoyancy-cavity workload itself: 128^3, Ra = 1e5, Pr = 0.71, walls
+    at +-0.5, seeded noise, 20000 steps, against the oracle's run of the same
+    case (tests/golden/thermal_full_128_s20000.npz: Nusselt numbers, residual
+    history, mid-line profiles and a strided subsample of the fields)."""
+    path = os.path.join(os.path.dirname(__file__), "golden", "thermal_full_128_s20000.npz")
+    d = np.load(path)
+    n, steps, stride = int(d["N"]), int(d["steps"]), int(d["stride"])
+    cfg = c2_like(n, n, n)
+    der = cfg.derived()
+    T0 = buoyancy_initial_temperature(n, n, n, seed=int(d["seed"]))
+    z = np.zeros_like(T0)
+    print(f"[gpu] thermal full {n}^3 steps={steps}", flush=True)
+    errs = []
+    with Cavity(cfg) as c:
+        c.load_fields(np.ones_like(T0), z, z, z, T0)
+        for _ in range(steps // 2000):
+            c.step(2000)
+            errs.append(c.check_thermal())
+        rho, u, v, w = c.macro()
+        T = c.temperature()
+        nu = c.nusselt()
+    U = float(np.sqrt(der["g_beta"] * 1.0 * n))
+    s = slice(stride // 2, None, stride)
+    h = n // 2
+    np.testing.assert_allclose(nu, (float(d["nu_vol"]), float(d["nu_wall"])), rtol=1e-11)
+    np.testing.assert_allclose(np.array(errs), d["errs"], rtol=1e-8)
+    assert relmax(rho[s, s, s], d["rho_sub"]) <= 1e-12
+    assert relmax(T[s, s, s], d["T_sub"]) <= 1e-11
+    for g, name in ((u, "u_sub"), (v, "v_sub"), (w, "w_sub")):
+        assert np.max(np.abs(g[s, s, s] - d[name])) <= 1e-11 * U, name
+    assert np.max(np.abs(T[h, :, h] - d["T_mid"])) <= 1e-11
+    assert abs(float(rho.sum()) - float(d["mass"])) / float(d["mass"]) <= 1e-13
