@@ -73,21 +73,26 @@ __global__ __launch_bounds__(256) void k_batch_step(const BatchArgs b) {
     const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
     const unsigned seed = *b.step_ctr;
+    int m = 0;  // instance of the current tile; tiles only move forward, so m does too
     for (int t = wave; t < b.total_tiles; t += nwaves) {
-        // instance of tile t: last m with tile_begin <= t
-        int lo = 0, hi = b.ninst - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (b.inst[mid].tile_begin <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        const int m = __builtin_amdgcn_readfirstlane(lo);
+        while (m + 1 < b.ninst && b.inst[m + 1].tile_begin <= t) ++m;
         const BatchInst& in = b.inst[m];
-        const long long lc = (long long)(t - in.tile_begin) * TILE + lane;
-        if (lc >= in.ncells) continue;
-        const int i = (int)(lc % in.nx);
-        const int j = (int)((lc / in.nx) % in.ny);
-        const int k = (int)(lc / ((long long)in.nx * in.ny));
+        // the tile's first cell, once per wave (uniform, 32-bit), then each
+        // lane steps forward by its lane number, carrying across rows
+        const unsigned nx = (unsigned)in.nx, ny = (unsigned)in.ny;
+        const unsigned lc0 = (unsigned)(t - in.tile_begin) * TILE;
+        unsigned k = lc0 / (nx * ny);
+        const unsigned rem = lc0 - k * nx * ny;
+        unsigned j = rem / nx;
+        unsigned i = rem - j * nx + (unsigned)lane;
+        while (i >= nx) {
+            i -= nx;
+            if (++j == ny) {
+                j = 0;
+                ++k;
+            }
+        }
+        if (k >= (unsigned)in.nz) continue;  // past the instance's last cell
         InstView a;
         a.src = static_cast<const T*>(b.src) + in.off;
         a.dst = static_cast<T*>(b.dst) + in.off;
@@ -109,7 +114,7 @@ __global__ __launch_bounds__(256) void k_batch_step(const BatchArgs b) {
         a.v = b.v + in.macro_off;
         a.w = b.w + in.macro_off;
         a.seed = seed;
-        cavity_cell<S, COLL, MACRO, false>(a, i, j, k);
+        cavity_cell<S, COLL, MACRO, false>(a, (int)i, (int)j, (int)k);
     }
 }
 
@@ -340,7 +345,8 @@ int lbm_batch_create(const lbm_params* params, int count, lbm_batch** out) {
     const lbm_params& p0 = params[0];
     for (int m = 0; m < count; ++m) {
         const lbm_params& p = params[m];
-        if (p.nx < 2 || p.ny < 2 || p.nz < 2)
+        // the tile walk indexes an instance's cells in 32 bits
+        if (p.nx < 2 || p.ny < 2 || p.nz < 2 || (long long)p.nx * p.ny * p.nz > (1ll << 31) - 64)
             return lbm_fail(LBM_ERR_ARG, "lbm_batch_create: instance %d grid %d x %d x %d", m, p.nx, p.ny, p.nz);
         if (!(p.tau > 0.5) || !std::isfinite(p.tau))
             return lbm_fail(LBM_ERR_TAU, "lbm_batch_create: instance %d tau = %g", m, p.tau);

@@ -64,19 +64,17 @@ struct StoreF32 {
     __device__ __forceinline__ static void st(T* p, Real v, float, unsigned) { __builtin_nontemporal_store(v, p); }
 };
 
-// FP32 -> IEEE binary16 with stochastic rounding: the 13 mantissa bits the
-// conversion drops are replaced by rounding up with probability equal to
-// their value (add a uniform 13-bit word to the magnitude, truncate).  The
-// result is exact in binary16 for the normal range; the final conversion
-// only rounds values in binary16's subnormal range.  Rounding to nearest
-// would round a slowly varying population the same way every step -- a
-// constant spurious force that the flow integrates over its long viscous
-// time; stochastic rounding makes that error unbiased.
+// FP32 -> IEEE binary16 with stochastic rounding, on gfx950's conversion
+// instruction (v_cvt_sr_f16_f32): the mantissa bits binary16 cannot hold are
+// rounded up with probability equal to their value, driven by a uniform
+// 32-bit word.  Rounding to nearest would round a slowly varying population
+// the same way every step -- a constant spurious force that the flow
+// integrates over its long viscous time; stochastic rounding makes that
+// error unbiased.
 __device__ __forceinline__ _Float16 to_half_stochastic(float x, unsigned rnd) {
-    unsigned b = __float_as_uint(x);
-    unsigned mag = b & 0x7fffffffu;
-    if (mag < 0x7f800000u) mag = (mag + (rnd & 0x1fffu)) & ~0x1fffu;
-    return static_cast<_Float16>(__uint_as_float((b & 0x80000000u) | mag));
+    typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+    const half2_t h = __builtin_amdgcn_cvt_sr_f16_f32(half2_t{0, 0}, x, rnd, false);
+    return h.x;
 }
 
 // Avalanche hash of a 32-bit word (the "lowbias32" integer mixer), used to
@@ -99,7 +97,10 @@ struct StoreF16 {
     static constexpr bool RANDOM = true;
     static constexpr int BYTES = 2;
     __device__ __forceinline__ static Real ld(const T* p, float inv_scale) {
-        return static_cast<float>(*p) * inv_scale;
+        // fma with +0 rather than a product: one mixed-precision fma
+        // (v_fma_mix_f32) instead of a conversion and a multiply; the value
+        // is the same (the scale is a power of two)
+        return __builtin_fmaf(static_cast<float>(*p), inv_scale, 0.0f);
     }
     __device__ __forceinline__ static void st(T* p, Real v, float scale, unsigned rnd) {
         __builtin_nontemporal_store(to_half_stochastic(v * scale, rnd), p);
@@ -107,9 +108,9 @@ struct StoreF16 {
 };
 
 // Random words for the stores of cell c at step `seed`: one avalanche hash
-// per cell, then a Weyl step per population (the conversion uses the top 13
-// bits).  Each word is uniform over cells and steps, which is what makes the
-// rounding unbiased; one hash per cell keeps the store ALU-cheap.
+// per cell, then a Weyl step per population.  Each word is uniform over cells
+// and steps, which is what makes the rounding unbiased; one hash per cell
+// keeps the store ALU-cheap.
 template <class S>
 __device__ __forceinline__ unsigned cell_random(long long c, unsigned seed) {
     if constexpr (S::RANDOM)
@@ -121,7 +122,7 @@ __device__ __forceinline__ unsigned cell_random(long long c, unsigned seed) {
 template <class S, int a>
 __device__ __forceinline__ unsigned store_random(unsigned h) {
     if constexpr (S::RANDOM)
-        return (h + (unsigned)a * 0x61C88647u) >> 19;
+        return h + (unsigned)a * 0x61C88647u;
     else
         return 0u;
 }

@@ -146,17 +146,6 @@ __device__ __forceinline__ double abs_pop(const typename S::T* buf, long long ps
     return S::SHIFT ? g + weight(a) : g;
 }
 
-// f_post of node (i,j,k) for an interpolation stencil: the rest equilibrium
-// w_a rho0 where the point is not a fluid node.
-template <class S>
-__device__ __forceinline__ double stencil_fpost(const PArgs& a, const typename S::T* src, int i, int j, int k, int al) {
-    if (!in_box(a, i, j, k)) return weight(al) * a.rho0;
-    const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
-    if (a.owner[m] >= 0) return weight(al) * a.rho0;
-    const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
-    return abs_pop<S>(src, a.ps, c, al, a.inv_scale);
-}
-
 // Bounding box (0-based cell indices, clamped) of sphere `id` with `margin`.
 __device__ __forceinline__ void sphere_box(const PArgs& a, const double* xc, int id, int margin, int lo[3], int hi[3]) {
     const int n[3] = {a.nx, a.ny, a.nz};
@@ -223,7 +212,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_link_scan(const PArgs a) {
     const int cap = a.slab_cap[id];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     constexpr int UNROLL = 4;
-    // link mask of box node t (0 when t is past the box or not a shell fluid node)
+    // link mask of box node t (0 when t is past the box or not a shell fluid
+    // node).  The node's and its 18 neighbours' owner words are loaded in one
+    // go (indices clamped into the box, out-of-box neighbours masked after),
+    // so a node costs one round trip to memory, not one per direction.
     auto links_at = [&](unsigned t, int& i, int& j, int& k) -> unsigned {
         const unsigned ti = t % bx, tj = (t / bx) % by, tk = t / (bx * by);
         i = lo[0] + (int)ti;
@@ -232,13 +224,19 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_link_scan(const PArgs a) {
         if (t >= total) return 0u;
         const double d2 = dist2(i + 1.0 - cx, j + 1.0 - cy, k + 1.0 - cz);
         if (!(d2 > rin2 && d2 <= rout2)) return 0u;
-        if (a.owner[i + (long long)a.nx * (j + (long long)a.ny * k)] >= 0) return 0u;
+        const int own = a.owner[i + (long long)a.nx * (j + (long long)a.ny * k)];
+        int nb[Q];
+        sfor<Q - 1>([&](auto B) {
+            constexpr int al = decltype(B)::value + 1;
+            const int ni = min(max(i + EX[al], 0), a.nx - 1), nj = min(max(j + EY[al], 0), a.ny - 1),
+                      nk = min(max(k + EZ[al], 0), a.nz - 1);
+            nb[al] = a.owner[ni + (long long)a.nx * (nj + (long long)a.ny * nk)];
+        });
+        if (own >= 0) return 0u;
         unsigned mk = 0;
         sfor<Q - 1>([&](auto B) {
             constexpr int al = decltype(B)::value + 1;
-            const int ni = i + EX[al], nj = j + EY[al], nk = k + EZ[al];
-            if (in_box(a, ni, nj, nk) && a.owner[ni + (long long)a.nx * (nj + (long long)a.ny * nk)] == id)
-                mk |= 1u << al;
+            if (in_box(a, i + EX[al], j + EY[al], k + EZ[al]) && nb[al] == id) mk |= 1u << al;
         });
         return mk;
     };
@@ -356,12 +354,29 @@ __global__ __launch_bounds__(256) void k_link_proc(const PArgs a) {
                       k = lo[2] + (int)((it >> 5) & 511u);
             const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
             // every value either branch of the interpolation may need, loaded
-            // before the bisection so the loads overlap it
+            // at once before the bisection (stencil points clamped into the
+            // box; points outside it or in a solid take the rest equilibrium
+            // after the loads), so one round trip to memory serves the link
+            const int i1 = i - EX[al], j1 = j - EY[al], k1 = k - EZ[al];
+            const int i2 = i1 - EX[al], j2 = j1 - EY[al], k2 = k1 - EZ[al];
+            const bool in1 = in_box(a, i1, j1, k1), in2 = in_box(a, i2, j2, k2);
+            const int ci1 = min(max(i1, 0), a.nx - 1), cj1 = min(max(j1, 0), a.ny - 1), ck1 = min(max(k1, 0), a.nz - 1);
+            const int ci2 = min(max(i2, 0), a.nx - 1), cj2 = min(max(j2, 0), a.ny - 1), ck2 = min(max(k2, 0), a.nz - 1);
+            const long long c1 = ci1 + (long long)a.px * (cj1 + (long long)a.ny * ck1);
+            const long long c2 = ci2 + (long long)a.px * (cj2 + (long long)a.ny * ck2);
+            const int o1 = a.owner[ci1 + (long long)a.nx * (cj1 + (long long)a.ny * ck1)];
+            const int o2 = a.owner[ci2 + (long long)a.nx * (cj2 + (long long)a.ny * ck2)];
             const double fa = abs_pop<S>(src, a.ps, c, al, a.inv_scale);
             const double fb = abs_pop<S>(src, a.ps, c, bd, a.inv_scale);
-            const double f1 = stencil_fpost<S>(a, src, i - EX[al], j - EY[al], k - EZ[al], al);
-            const double f2 = stencil_fpost<S>(a, src, i - 2 * EX[al], j - 2 * EY[al], k - 2 * EZ[al], al);
-            const double fb1 = stencil_fpost<S>(a, src, i - EX[al], j - EY[al], k - EZ[al], bd);
+            const double s1a = abs_pop<S>(src, a.ps, c1, al, a.inv_scale);
+            const double s2a = abs_pop<S>(src, a.ps, c2, al, a.inv_scale);
+            const double s1b = abs_pop<S>(src, a.ps, c1, bd, a.inv_scale);
+            // f_post at the stencil points, the rest equilibrium w rho0 where
+            // the point is not a fluid node
+            const bool fl1 = in1 && o1 < 0, fl2 = in2 && o2 < 0;
+            const double f1 = fl1 ? s1a : weight(al) * a.rho0;
+            const double f2 = fl2 ? s2a : weight(al) * a.rho0;
+            const double fb1 = fl1 ? s1b : weight(bd) * a.rho0;
             const double cx = a.xc[3 * id], cy = a.xc[3 * id + 1], cz = a.xc[3 * id + 2];
             double x0, y0, z0, qq;
             if (!calq(cx, cy, cz, a.r[id], i + 1.0, j + 1.0, k + 1.0, EX[al], EY[al], EZ[al], x0, y0, z0, qq))

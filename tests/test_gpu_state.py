@@ -47,7 +47,9 @@ def test_cavity_restart_is_exact(storage):
 
 
 def test_cavity_restart_with_halo_loopback_is_exact():
-    restart_matches(CavityConfig(32, 30, 28, Reynolds=300, storage="f32", halo_faces=0x3F, halo_loopback=1), 50, 40)
+    # x and y faces exchange (periodic through the loopback); the z walls and
+    # the lid keep a flow going, so the residual is defined
+    restart_matches(CavityConfig(32, 30, 28, Reynolds=300, storage="f32", halo_faces=0x0F, halo_loopback=1), 50, 40)
 
 
 def test_thermal_restart_is_exact():
