@@ -40,6 +40,15 @@ CASES = [
     (12, 10, 9, 300),
     (16, 16, 16, 500),
     (20, 17, 13, 1000),
+    # edge shapes: the smallest grid (every node touches a wall and the lid),
+    # two-node-thin slabs along each axis, and x extents around one and two
+    # wavefronts (65: a one-lane tail; 33: half a wave)
+    (2, 2, 2, 50),
+    (2, 7, 3, 40),
+    (3, 2, 5, 60),
+    (4, 5, 2, 60),
+    (33, 17, 5, 200),
+    (65, 3, 4, 100),
 ]
 
 THERMAL_CASES = [
@@ -48,6 +57,12 @@ THERMAL_CASES = [
     (12, 10, 9, 300),
     (16, 16, 16, 500),
     (20, 17, 13, 1000),
+    # edge shapes: three nodes between the held walls, two-node-thin slabs,
+    # an x extent past one wavefront
+    (3, 3, 3, 60),
+    (2, 5, 4, 80),
+    (5, 4, 2, 80),
+    (65, 6, 5, 100),
 ]
 
 
