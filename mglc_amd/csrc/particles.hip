@@ -251,14 +251,15 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_link_scan(const PArgs a) {
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) cnt += __popc(mk[u]);
     }
-    int incl = cnt;
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += y;
-    }
-    if (lane == 63) wave_tot[wid] = incl;
+    // each wave owns a contiguous part of the slab (waves in order); inside
+    // it the entries go pass by pass, direction by direction, lane by lane:
+    // 64 consecutive entries share a direction and run along x, so the
+    // link pass's gathers from the same population array fall on few lines
+    int wcnt = cnt;
+    for (int off = 32; off > 0; off >>= 1) wcnt += __shfl_down(wcnt, off, 64);
+    if (lane == 0) wave_tot[wid] = wcnt;
     __syncthreads();
-    int pos = incl - cnt, all = 0;
+    int pos = 0, all = 0;
     for (int w = 0; w < SCAN_THREADS / 64; ++w) {
         if (w < wid) pos += wave_tot[w];
         all += wave_tot[w];
@@ -277,12 +278,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_link_scan(const PArgs a) {
         for (int u = 0; u < UNROLL; ++u) mk[u] = links_at(base + u * SCAN_THREADS + threadIdx.x, ii[u], jj[u], kk[u]);
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
-            unsigned m = mk[u];
-            while (m) {
-                const int al = __ffs(m) - 1;
-                m &= m - 1;
-                list[pos++] = link_entry(ii[u] - lo[0], jj[u] - lo[1], kk[u] - lo[2], al);
-            }
+            const unsigned e0 = link_entry(ii[u] - lo[0], jj[u] - lo[1], kk[u] - lo[2], 0);
+            sfor<Q - 1>([&](auto B) {
+                constexpr int al = decltype(B)::value + 1;
+                const bool has = (mk[u] >> al) & 1u;
+                const unsigned long long bal = __ballot(has);
+                const int before = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+                if (has) list[pos + before] = e0 | (unsigned)al;
+                pos += (int)__popcll(bal);
+            });
         }
     }
     if (threadIdx.x == 0) a.link_count[id] = all;
@@ -353,10 +358,16 @@ __global__ __launch_bounds__(256) void k_link_proc(const PArgs a) {
             const int i = lo[0] + (int)(it >> 23), j = lo[1] + (int)((it >> 14) & 511u),
                       k = lo[2] + (int)((it >> 5) & 511u);
             const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
-            // every value either branch of the interpolation may need, loaded
-            // at once before the bisection (stencil points clamped into the
-            // box; points outside it or in a solid take the rest equilibrium
-            // after the loads), so one round trip to memory serves the link
+            // the wall distance first: it decides which three populations the
+            // interpolation reads, so each link gathers three values from the
+            // source buffer (the pass is bound by these scattered reads)
+            const double cx = a.xc[3 * id], cy = a.xc[3 * id + 1], cz = a.xc[3 * id + 2];
+            double x0, y0, z0, qq;
+            if (!calq(cx, cy, cz, a.r[id], i + 1.0, j + 1.0, k + 1.0, EX[al], EY[al], EZ[al], x0, y0, z0, qq))
+                stop(a, 3);
+            const bool near = qq < 0.5;
+            // stencil points x - e_a and x - 2 e_a, clamped into the box; a
+            // point outside it or in a solid takes the rest equilibrium w rho0
             const int i1 = i - EX[al], j1 = j - EY[al], k1 = k - EZ[al];
             const int i2 = i1 - EX[al], j2 = j1 - EY[al], k2 = k1 - EZ[al];
             const bool in1 = in_box(a, i1, j1, k1), in2 = in_box(a, i2, j2, k2);
@@ -366,30 +377,24 @@ __global__ __launch_bounds__(256) void k_link_proc(const PArgs a) {
             const long long c2 = ci2 + (long long)a.px * (cj2 + (long long)a.ny * ck2);
             const int o1 = a.owner[ci1 + (long long)a.nx * (cj1 + (long long)a.ny * ck1)];
             const int o2 = a.owner[ci2 + (long long)a.nx * (cj2 + (long long)a.ny * ck2)];
+            // q < 1/2: f_post(a) at x, x - e_a, x - 2 e_a; q >= 1/2: f_post(a)
+            // and f_post(abar) at x, f_post(abar) at x - e_a
             const double fa = abs_pop<S>(src, a.ps, c, al, a.inv_scale);
-            const double fb = abs_pop<S>(src, a.ps, c, bd, a.inv_scale);
-            const double s1a = abs_pop<S>(src, a.ps, c1, al, a.inv_scale);
-            const double s2a = abs_pop<S>(src, a.ps, c2, al, a.inv_scale);
-            const double s1b = abs_pop<S>(src, a.ps, c1, bd, a.inv_scale);
-            // f_post at the stencil points, the rest equilibrium w rho0 where
-            // the point is not a fluid node
+            const double s1 = abs_pop<S>(src, a.ps, c1, near ? al : bd, a.inv_scale);
+            const double s2 = abs_pop<S>(src, a.ps, near ? c2 : c, near ? al : bd, a.inv_scale);
             const bool fl1 = in1 && o1 < 0, fl2 = in2 && o2 < 0;
-            const double f1 = fl1 ? s1a : weight(al) * a.rho0;
-            const double f2 = fl2 ? s2a : weight(al) * a.rho0;
-            const double fb1 = fl1 ? s1b : weight(bd) * a.rho0;
-            const double cx = a.xc[3 * id], cy = a.xc[3 * id + 1], cz = a.xc[3 * id + 2];
-            double x0, y0, z0, qq;
-            if (!calq(cx, cy, cz, a.r[id], i + 1.0, j + 1.0, k + 1.0, EX[al], EY[al], EZ[al], x0, y0, z0, qq))
-                stop(a, 3);
             double uwx, uwy, uwz;
             body_velocity(a.xc, a.U, a.Om, id, x0, y0, z0, uwx, uwy, uwz);
             const double eu = EX[bd] * uwx + EY[bd] * uwy + EZ[bd] * uwz;
             const double rho_avg = a.rho_avg[0];
             double val;
-            if (qq < 0.5)
+            if (near) {
+                const double f1 = fl1 ? s1 : weight(al) * a.rho0;
+                const double f2 = fl2 ? s2 : weight(al) * a.rho0;
                 val = qq * (1.0 + 2.0 * qq) * fa + (1.0 - 4.0 * qq * qq) * f1 - qq * (1.0 - 2.0 * qq) * f2 +
                       6.0 * weight(al) * rho_avg * eu;
-            else {
+            } else {
+                const double fb = s2, fb1 = fl1 ? s1 : weight(bd) * a.rho0;
                 const double iq = 1.0 / qq, i12 = 1.0 / (1.0 + 2.0 * qq);
                 val = fa * iq * i12 + fb * (2.0 * qq - 1.0) * iq - fb1 * (2.0 * qq - 1.0) * i12 +
                       6.0 * weight(al) * rho_avg * iq * i12 * eu;
