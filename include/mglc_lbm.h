@@ -159,6 +159,10 @@ int lbm_nusselt(lbm_ctx *ctx, double *nu_vol, double *nu_wall);
 
 /* Copy T of the last step (thermal contexts) to a host array of nx*ny*nz. */
 int lbm_download_temperature(lbm_ctx *ctx, double *T);
+/* Volume Reynolds number of the last macro fields, as calNuRe
+ * (RaNu.F90:36-44): sqrt(<u^2 + v^2 + w^2>) * nz / nu with nu = (tau - 1/2)/3,
+ * FP64, deterministic.  Any context without spheres. */
+int lbm_reynolds(lbm_ctx *ctx, double *re);
 
 /* The two global sums behind errorU (check.f90:12-17 before the Allreduce of
  * check.f90:25-28): e1 = sum (u-up)^2 + (v-vp)^2, e2 = sum u^2 + v^2 + w^2 over

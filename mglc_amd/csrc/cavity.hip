@@ -724,6 +724,13 @@ int lbm_nusselt(lbm_ctx* c, double* nu_vol, double* nu_wall) {
     return thermal_nusselt(c, nu_vol, nu_wall);
 }
 
+int lbm_reynolds(lbm_ctx* c, double* re) {
+    if (!c || !re) return lbm_fail(LBM_ERR_ARG, "lbm_reynolds: null argument");
+    if (!c->initialised) return lbm_fail(LBM_ERR_STATE, "lbm_reynolds before lbm_initial");
+    if (c->part) return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_reynolds: the macro fields of a particle run hold solids");
+    return flow_reynolds(c, re);
+}
+
 int lbm_download_temperature(lbm_ctx* c, double* T) {
     if (!c || !T) return lbm_fail(LBM_ERR_ARG, "lbm_download_temperature: null argument");
     if (!c->p.thermal) return lbm_fail(LBM_ERR_STATE, "lbm_download_temperature on an isothermal context");

@@ -66,6 +66,7 @@ int thermal_load(lbm_ctx* c, const double* d_rho, const double* d_u, const doubl
 int thermal_step(lbm_ctx* c, int nsteps);
 int thermal_check(lbm_ctx* c, double* sums4);
 int thermal_nusselt(lbm_ctx* c, double* nu_vol, double* nu_wall);
+int flow_reynolds(lbm_ctx* c, double* re);
 
 // Particle path (particles.hip).
 int particles_step(lbm_ctx* c, int nsteps);

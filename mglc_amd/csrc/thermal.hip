@@ -376,6 +376,30 @@ int thermal_check(lbm_ctx* c, double* sums4) {
     return finish_sums<4>(c, sums4);
 }
 
+// Sum of |u|^2 over all cells (calNuRe's Reynolds sum, RaNu.F90:36-44).
+__global__ __launch_bounds__(RED_THREADS) void k_kinetic(const double* __restrict__ u, const double* __restrict__ v,
+                                                         const double* __restrict__ w, long long n,
+                                                         double* __restrict__ partial) {
+    double e[1] = {0.0};
+    for (long long c = blockIdx.x * (long long)RED_THREADS + threadIdx.x; c < n;
+         c += (long long)gridDim.x * RED_THREADS)
+        e[0] += u[c] * u[c] + v[c] * v[c] + w[c] * w[c];
+    block_store<1>(e, partial);
+}
+
+int flow_reynolds(lbm_ctx* c, double* re) {
+    hipLaunchKernelGGL(k_kinetic, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->macro + c->n,
+                       c->macro + 2 * c->n, c->macro + 3 * c->n, c->n, c->red);
+    int rc = lbm_check_launch();
+    if (rc) return rc;
+    double s[1];
+    rc = finish_sums<1>(c, s);
+    if (rc) return rc;
+    const double viscosity = (c->p.tau - 0.5) / 3.0;
+    *re = std::sqrt(s[0] / (double)c->n) * (double)c->p.nz / viscosity;
+    return LBM_OK;
+}
+
 int thermal_nusselt(lbm_ctx* c, double* nu_vol, double* nu_wall) {
     const int ax = c->p.thermal_axis;
     if (ax < 0 || ax > 2) return lbm_fail(LBM_ERR_ARG, "lbm_nusselt: no held walls (thermal_axis = %d)", ax);

@@ -35,7 +35,7 @@ EXPORTED_SYMBOLS = (
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_initial", "lbm_batch_step", "lbm_batch_step_each",
     "lbm_batch_autotune", "lbm_batch_set_launch", "lbm_batch_synchronize", "lbm_batch_download_macro",
     "lbm_batch_get_info", "lbm_particles_init", "lbm_particles_get", "lbm_particles_map",
-    "lbm_state_bytes", "lbm_save_state", "lbm_load_state",
+    "lbm_state_bytes", "lbm_save_state", "lbm_load_state", "lbm_reynolds",
 )
 
 FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = range(6)
@@ -134,6 +134,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lbm_particles_init.argtypes = [vp, ctypes.POINTER(LbmParticleParams), dp, dp, dp, dp]
     lib.lbm_particles_get.argtypes = [vp, dp, dp, dp, dp, dp]
     lib.lbm_particles_map.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    lib.lbm_reynolds.argtypes = [vp, dp]
     lib.lbm_state_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
     lib.lbm_save_state.argtypes = [vp, vp, ctypes.c_size_t]
     lib.lbm_load_state.argtypes = [vp, vp, ctypes.c_size_t]
@@ -395,6 +396,12 @@ class Cavity:
         a, b = ctypes.c_double(), ctypes.c_double()
         self._call("lbm_nusselt", self._ctx, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def reynolds(self) -> float:
+        """Volume Reynolds number sqrt(<|u|^2>) nz / nu (calNuRe, RaNu.F90:36-44)."""
+        r = ctypes.c_double()
+        self._call("lbm_reynolds", self._ctx, ctypes.byref(r))
+        return r.value
 
     def temperature(self):
         T = np.empty(self.shape, dtype=np.float64)

@@ -161,3 +161,25 @@ def test_full_size_buoyancy_cavity_against_oracle_fixture():
         assert np.max(np.abs(g[s, s, s] - d[name])) <= 1e-11 * U, name
     assert np.max(np.abs(T[h, :, h] - d["T_mid"])) <= 1e-11
     assert abs(float(rho.sum()) - float(d["mass"])) / float(d["mass"]) <= 1e-13
+
+
+def test_reynolds_number_against_the_oracle_fields(oracle_lib):
+    """lbm_reynolds = sqrt(<|u|^2>) nz / nu (RaNu.F90:36-44), from the GPU's
+    FP64 sums, against the same formula on the oracle's fields."""
+    O = oracle_lib
+    nx, ny, nz, steps = 22, 20, 18, 600
+    cfg = c2_like(nx, ny, nz)
+    d = cfg.derived()
+    T0 = buoyancy_initial_temperature(nx, ny, nz, seed=2024)
+    z = np.zeros_like(T0)
+    fields = (np.ones_like(T0), z, z, z, T0)
+    p = O.thermal_params(nx, ny, nz, d["tau"], d["para_a"], d["g_beta"], 0.0, 0.5, -0.5, 0.0, tconst_axis=1)
+    ref = O.thermal_run(p, steps, fields=fields)
+    print(f"[gpu] reynolds {nx}x{ny}x{nz} steps={steps}", flush=True)
+    with Cavity(cfg) as c:
+        c.load_fields(*fields)
+        c.step(steps)
+        re = c.reynolds()
+    u, v, w = ref[1], ref[2], ref[3]
+    expect = float(np.sqrt(np.mean(u * u + v * v + w * w)) * nz / d["viscosity"])
+    assert re > 0 and abs(re - expect) / expect <= 1e-10
