@@ -41,15 +41,22 @@ def build_library() -> str:
     return LIB
 
 
-def build_fortran_driver() -> str:
-    """mglc_cavity: the Fortran program that keeps the reference's parameter
-    module and loop and calls the library through iso_c_binding."""
+def build_fortran_driver() -> list[str]:
+    """The Fortran programs that keep the reference's parameter modules and
+    loops and call the library through iso_c_binding: mglc_cavity (the
+    lid-driven cavity) and mglc_buoyancy (the buoyancy-driven cavity)."""
     os.makedirs(BIN_DIR, exist_ok=True)
     mod_dir = os.path.join(BIN_DIR, "mod")
     os.makedirs(mod_dir, exist_ok=True)
-    order = ["mglc_lbm_capi.f90", "commondata.f90", "output.f90", "main.f90"]
-    srcs = [os.path.join(FORTRAN_DIR, f) for f in order]
-    exe = os.path.join(BIN_DIR, "mglc_cavity")
-    _run([FC, "-O2", "-J", mod_dir, *srcs, "-o", exe, f"-L{HERE}", "-lmglc_lbm",
-          f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/.."])
-    return exe
+    programs = {
+        "mglc_cavity": ["mglc_lbm_capi.f90", "commondata.f90", "output.f90", "main.f90"],
+        "mglc_buoyancy": ["mglc_lbm_capi.f90", "buoyancy_data.f90", "output.f90", "buoyancy_main.f90"],
+    }
+    exes = []
+    for name, order in programs.items():
+        srcs = [os.path.join(FORTRAN_DIR, f) for f in order]
+        exe = os.path.join(BIN_DIR, name)
+        _run([FC, "-O2", "-J", mod_dir, *srcs, "-o", exe, f"-L{HERE}", "-lmglc_lbm",
+              f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/.."])
+        exes.append(exe)
+    return exes

@@ -1,0 +1,253 @@
+! buoyancy_data.f90 -- run parameters of the buoyancy-driven cavity driver,
+! under the names of the reference's module (MPI/Buoyancy_driven_cavity/
+! fortran/3d/mpi_blocked/module.F90:1-80).  The reference fixes them as
+! compile-time parameters; here they keep its values as defaults and can be
+! overridden from a namelist file (&buoyancy ... /), and the derived
+! constants follow its parameter statements (module.F90:19-33).
+module buoyancy_data
+    implicit none
+
+    integer :: total_nx = 51, total_ny = 51, total_nz = 51
+    integer :: nx, ny, nz
+    real(8) :: Rayleigh = 1.0d6
+    real(8) :: Prandtl = 0.71d0
+    real(8) :: Mach = 0.1d0
+    real(8) :: Thot = 1.0d0, Tcold = 0.0d0, Tref = 0.0d0
+    real(8) :: Ekman = 0.001d0               ! <= 0: no Coriolis force
+    integer :: dimensionlessTimeMax = 1000
+    integer :: flowReversal = 0               ! 1: convergence waits flowReversalTime units
+    integer :: flowReversalTime = 20000
+    real(8) :: epsU = 1e-8, epsT = 1e-8
+    integer :: check_every = 2000             ! main.F90: check() every 2000 steps
+    logical :: statistics = .false.           ! calNuRe every int(timeUnit) steps (RaNu.F90)
+    integer :: heated_axis = 1                ! walls held at Thot / Tcold: 0 x, 1 y, 2 z
+    real(8) :: noise_amplitude = 0.0d0        ! seeded uniform noise on the initial T
+    integer :: noise_seed = 2024
+    integer :: itc_limit = -1                 ! >= 0: overrides itc_max
+    character(len=8) :: storage = 'F64'       ! the buoyancy path computes and stores FP64
+    character(len=256) :: output_dir = '.'
+    logical :: write_tecplot = .true.
+    logical :: write_binary = .false.
+    logical :: write_raw = .false.
+    character(len=256) :: restart_file = ''
+    character(len=256) :: backup_file = ''
+
+    ! derived (module.F90:19-33)
+    real(8) :: tauf, viscosity, diffusivity, omegaRatating, paraA, gBeta1, gBeta, timeUnit
+    integer :: itc, itc_max
+    real(8) :: errorU, errorT
+
+    real(8), allocatable :: xp(:), yp(:), zp(:)
+    real(8), allocatable :: u(:, :, :), v(:, :, :), w(:, :, :), T(:, :, :), rho(:, :, :)
+
+    ! calNuRe state (module.F90:62-66); the series grow as needed
+    integer :: dimensionlessTime = 0
+    integer :: statisticallyStationaryState = 0, statisticallyStationaryTime = 0
+    integer :: statisticallyConvergeState = 0
+    real(8), allocatable :: NuVolAvg(:), ReVolAvg(:), NuVolAvg_mean(:), ReVolAvg_mean(:)
+
+    namelist /buoyancy/ total_nx, total_ny, total_nz, Rayleigh, Prandtl, Mach, Thot, Tcold, Tref, Ekman, &
+                        dimensionlessTimeMax, flowReversal, flowReversalTime, epsU, epsT, check_every, &
+                        statistics, heated_axis, noise_amplitude, noise_seed, itc_limit, storage, output_dir, &
+                        write_tecplot, write_binary, write_raw, restart_file, backup_file
+
+contains
+
+    subroutine read_parameters()
+        character(len=512) :: path
+        integer :: ios, unit
+        if (command_argument_count() >= 1) then
+            call get_command_argument(1, path)
+            open(newunit=unit, file=trim(path), status='old', action='read', iostat=ios)
+            if (ios /= 0) then
+                write(*,*) 'error: cannot open parameter file ', trim(path)
+                error stop 1
+            endif
+            read(unit, nml=buoyancy, iostat=ios)
+            if (ios /= 0) then
+                write(*,*) 'error: bad &buoyancy namelist in ', trim(path)
+                error stop 1
+            endif
+            close(unit)
+        endif
+        nx = total_nx
+        ny = total_ny
+        nz = total_nz
+        tauf = 0.5d0 + Mach * dble(total_nz) * dsqrt(3.0d0 * Prandtl / Rayleigh)
+        viscosity = (tauf - 0.5d0) / 3.0d0
+        diffusivity = viscosity / Prandtl
+        if (Ekman > 0.0d0) then
+            omegaRatating = viscosity / 2.0d0 / Ekman / dble(total_nz * total_nz)
+        else
+            omegaRatating = 0.0d0
+        endif
+        paraA = 42.0d0 * dsqrt(3.0d0) * diffusivity - 6.0d0
+        gBeta1 = Rayleigh * viscosity * diffusivity / dble(total_nz)
+        gBeta = gBeta1 / dble(total_nz * total_nz)
+        timeUnit = dsqrt(dble(total_nz) / gBeta)
+        itc_max = int(dimensionlessTimeMax * timeUnit)
+        if (itc_limit >= 0) itc_max = itc_limit
+    end subroutine read_parameters
+
+    ! calNuRe (RaNu.F90:1-137) on the values of the current fields: Nu_vol
+    ! and Re_vol come from lbm_nusselt / lbm_reynolds.  Appends the series to
+    ! Nu_Bulk.dat / Re_Bulk.dat and the stationarity / convergence notes to
+    ! statistically.txt, and sets statisticallyConvergeState.
+    subroutine calNuRe(nu_vol, re_vol)
+        real(8), intent(in) :: nu_vol, re_vol
+        integer :: q, halfTime, unit
+        real(8) :: nf, rf, nl, rl, dn, dr
+        real(8), allocatable :: grow(:)
+
+        dimensionlessTime = dimensionlessTime + 1
+        if (dimensionlessTime > ubound(NuVolAvg, 1)) then
+            q = 2 * ubound(NuVolAvg, 1) + 1
+            allocate(grow(0:q)); grow = 0.0d0; grow(0:ubound(NuVolAvg, 1)) = NuVolAvg; call move_alloc(grow, NuVolAvg)
+            allocate(grow(0:q)); grow = 0.0d0; grow(0:ubound(ReVolAvg, 1)) = ReVolAvg; call move_alloc(grow, ReVolAvg)
+            allocate(grow(0:q)); grow = 0.0d0; grow(0:ubound(NuVolAvg_mean, 1)) = NuVolAvg_mean
+            call move_alloc(grow, NuVolAvg_mean)
+            allocate(grow(0:q)); grow = 0.0d0; grow(0:ubound(ReVolAvg_mean, 1)) = ReVolAvg_mean
+            call move_alloc(grow, ReVolAvg_mean)
+        endif
+        NuVolAvg(dimensionlessTime) = nu_vol
+        ReVolAvg(dimensionlessTime) = re_vol
+        NuVolAvg_mean(dimensionlessTime) = sum(NuVolAvg(1:dimensionlessTime)) / dble(dimensionlessTime)
+        ReVolAvg_mean(dimensionlessTime) = sum(ReVolAvg(1:dimensionlessTime)) / dble(dimensionlessTime)
+
+        open(newunit=unit, file=trim(output_dir)//'/Nu_Bulk.dat', status='unknown', position='append')
+        write(unit, *) itc / timeUnit, nu_vol
+        close(unit)
+        open(newunit=unit, file=trim(output_dir)//'/Re_Bulk.dat', status='unknown', position='append')
+        write(unit, *) itc / timeUnit, re_vol
+        close(unit)
+
+        if (statisticallyStationaryState == 1) then
+            if (mod(dimensionlessTime - statisticallyStationaryTime, 200) == 0) then
+                halfTime = (statisticallyStationaryTime + dimensionlessTime) / 2
+                nf = sum(NuVolAvg(statisticallyStationaryTime:halfTime)) / dble(halfTime - statisticallyStationaryTime + 1)
+                rf = sum(ReVolAvg(statisticallyStationaryTime:halfTime)) / dble(halfTime - statisticallyStationaryTime + 1)
+                nl = sum(NuVolAvg(halfTime:dimensionlessTime)) / dble(dimensionlessTime - halfTime + 1)
+                rl = sum(ReVolAvg(halfTime:dimensionlessTime)) / dble(dimensionlessTime - halfTime + 1)
+                dn = dabs((nf - nl) / nl)
+                dr = dabs((rf - rl) / rl)
+                open(newunit=unit, file=trim(output_dir)//'/statistically.txt', status='unknown', position='append')
+                if (dn <= 0.01d0 .and. dr <= 0.01d0) then
+                    if (flowReversal == 0) then
+                        statisticallyConvergeState = 1
+                    else if (dimensionlessTime >= statisticallyStationaryTime + flowReversalTime) then
+                        statisticallyConvergeState = 1
+                    endif
+                    write(unit, *) 'Statistically Converge Reached at Time =', dimensionlessTime
+                endif
+                write(unit, *) 'statisticallyStationaryTime=', statisticallyStationaryTime
+                write(unit, *) 'halfTime =', halfTime
+                write(unit, *) 'dimensionlessTime =', dimensionlessTime
+                write(unit, *) 'NuVolAvg diff. (first and last half):', dn * 100.0d0, '%'
+                write(unit, *) 'ReVolAvg diff. (first and last half):', dr * 100.0d0, '%'
+                close(unit)
+            endif
+        endif
+
+        if (statisticallyStationaryState == 0) then
+            if (mod(dimensionlessTime, 100) == 0) then
+                dn = dabs((NuVolAvg_mean(dimensionlessTime) - NuVolAvg_mean(dimensionlessTime - 100)) / &
+                          NuVolAvg_mean(dimensionlessTime))
+                dr = dabs((ReVolAvg_mean(dimensionlessTime) - ReVolAvg_mean(dimensionlessTime - 100)) / &
+                          ReVolAvg_mean(dimensionlessTime))
+                open(newunit=unit, file=trim(output_dir)//'/statistically.txt', status='unknown', position='append')
+                if (dn <= 0.01d0 .and. dr <= 0.01d0) then
+                    statisticallyStationaryState = 1
+                    statisticallyStationaryTime = dimensionlessTime
+                    write(unit, *) 'Statistically Stationary Reached at Time =', statisticallyStationaryTime
+                endif
+                write(unit, *) 'dimensionlessTime =', dimensionlessTime
+                write(unit, *) 'NuVolAvg diff. (100 timeunit interval):', dn * 100.0d0, '%'
+                write(unit, *) 'ReVolAvg diff. (100 timeunit interval):', dr * 100.0d0, '%'
+                close(unit)
+            endif
+        endif
+    end subroutine calNuRe
+
+    ! a * b modulo 2^64 on 16-bit limbs (no signed overflow)
+    pure function mul64(a, b) result(r)
+        integer(8), intent(in) :: a, b
+        integer(8) :: r, al(0:3), bl(0:3), col(0:3), carry, t
+        integer :: i, j
+        do i = 0, 3
+            al(i) = iand(ishft(a, -16 * i), 65535_8)
+            bl(i) = iand(ishft(b, -16 * i), 65535_8)
+        enddo
+        col = 0_8
+        do i = 0, 3
+            do j = 0, 3 - i
+                col(i + j) = col(i + j) + al(i) * bl(j)
+            enddo
+        enddo
+        r = 0_8
+        carry = 0_8
+        do i = 0, 3
+            t = col(i) + carry
+            r = ior(r, ishft(iand(t, 65535_8), 16 * i))
+            carry = ishft(t, -16)
+        enddo
+    end function mul64
+
+    ! uniform in [-amp, amp) from SplitMix64 of (index + seed): the seeded
+    ! noise of the buoyancy workload, identical to mglc_amd.lbm.splitmix_uniform
+    function splitmix_uniform(index, seed, amp) result(x)
+        integer(8), intent(in) :: index
+        integer, intent(in) :: seed
+        real(8), intent(in) :: amp
+        real(8) :: x
+        integer(8) :: z
+        integer(8), parameter :: golden = -7046029254386353131_8   ! 0x9E3779B97F4A7C15
+        integer(8), parameter :: m1 = -4658895280553007687_8       ! 0xBF58476D1CE4E5B9
+        integer(8), parameter :: m2 = -7723592293110705685_8       ! 0x94D049BB133111EB
+        z = mul64(index + int(seed, 8), golden)
+        z = mul64(ieor(z, ishft(z, -30)), m1)
+        z = mul64(ieor(z, ishft(z, -27)), m2)
+        z = ieor(z, ishft(z, -31))
+        x = amp * (2.0d0 * (dble(ishft(z, -11)) * (1.0d0 / 9007199254740992.0d0)) - 1.0d0)
+    end function splitmix_uniform
+
+    ! initial.F90:126-162: T = 0 with the layers next to the held walls at
+    ! Thot / Tcold; with noise_amplitude > 0 every other node gets seeded noise
+    ! (index over the nodes between the held layers, x fastest)
+    subroutine initial_temperature()
+        integer :: i, j, k, q, nq
+        integer(8) :: idx
+        integer :: n(0:2), c(0:2)
+        n = [nx, ny, nz]
+        nq = n(heated_axis)
+        idx = 0_8
+        do k = 1, nz
+            do j = 1, ny
+                do i = 1, nx
+                    c = [i, j, k]
+                    q = c(heated_axis)
+                    if (q == 1) then
+                        T(i, j, k) = Thot
+                    else if (q == nq) then
+                        T(i, j, k) = Tcold
+                    else
+                        T(i, j, k) = 0.0d0
+                    endif
+                enddo
+            enddo
+        enddo
+        if (noise_amplitude > 0.0d0) then
+            do k = 1, nz
+                do j = 1, ny
+                    do i = 1, nx
+                        c = [i, j, k]
+                        q = c(heated_axis)
+                        if (q == 1 .or. q == nq) cycle
+                        T(i, j, k) = splitmix_uniform(idx, noise_seed, noise_amplitude)
+                        idx = idx + 1_8
+                    enddo
+                enddo
+            enddo
+        endif
+    end subroutine initial_temperature
+
+end module buoyancy_data

@@ -22,7 +22,6 @@ program main
     integer(8) :: t_start, t_end, t_rate
     real(8) :: wall
     character(len=16) :: arg1
-    type(lbm_info) :: info
 
     if (command_argument_count() >= 1) then
         call get_command_argument(1, arg1)
@@ -93,10 +92,7 @@ program main
     allocate(u(nx, ny, nz), v(nx, ny, nz), w(nx, ny, nz), rho(nx, ny, nz))
     if (len_trim(restart_file) > 0) then
         ! the restart path of the reference (loadInitField): continue a saved run
-        call read_state(trim(restart_file))
-        call lbm_require(lbm_get_info(ctx, info), 'lbm_get_info')
-        itc = int(info%steps_done)
-        write(*,*) "restarted from ", trim(restart_file), " at itc =", itc
+        call lbm_load_state_file(ctx, trim(restart_file), itc)
     else
         call lbm_require(lbm_initial(ctx), 'lbm_initial')
         itc = 0
@@ -134,50 +130,10 @@ program main
     if (write_binary) call output_binary(output_dir, u, v, rho, nx, ny, nz, itc)
     if (write_raw) call output_raw(output_dir, rho, u, v, w, nx, ny, nz, itc)
 
-    if (len_trim(backup_file) > 0) call write_state(trim(backup_file))
+    if (len_trim(backup_file) > 0) call lbm_save_state_file(ctx, trim(backup_file))
 
     call lbm_require(lbm_destroy(ctx), 'lbm_destroy')
     deallocate(u, v, w, rho, xp, yp, zp)
     write(*,*) "Successfully: DNS completed!"
 
-contains
-
-    ! backupData (seq/bouyancy3d.F90:1011-1030): the whole device state as one
-    ! stream file
-    subroutine write_state(path)
-        character(len=*), intent(in) :: path
-        integer(c_size_t) :: bytes
-        integer(c_int8_t), allocatable :: blob(:)
-        integer :: unit
-        call lbm_require(lbm_state_bytes(ctx, bytes), 'lbm_state_bytes')
-        allocate(blob(bytes))
-        call lbm_require(lbm_save_state(ctx, blob, bytes), 'lbm_save_state')
-        open(newunit=unit, file=path, access='stream', form='unformatted', status='replace', action='write')
-        write(unit) blob
-        close(unit)
-        deallocate(blob)
-        write(*,*) "state saved to ", path
-    end subroutine write_state
-
-    subroutine read_state(path)
-        character(len=*), intent(in) :: path
-        integer(c_size_t) :: bytes
-        integer(c_int8_t), allocatable :: blob(:)
-        integer :: unit, ios
-        call lbm_require(lbm_state_bytes(ctx, bytes), 'lbm_state_bytes')
-        allocate(blob(bytes))
-        open(newunit=unit, file=path, access='stream', form='unformatted', status='old', action='read', iostat=ios)
-        if (ios /= 0) then
-            write(*,*) 'error: cannot open restart file ', path
-            error stop 1
-        endif
-        read(unit, iostat=ios) blob
-        close(unit)
-        if (ios /= 0) then
-            write(*,*) 'error: restart file ', path, ' is shorter than this run''s state'
-            error stop 1
-        endif
-        call lbm_require(lbm_load_state(ctx, blob, bytes), 'lbm_load_state')
-        deallocate(blob)
-    end subroutine read_state
 end program main

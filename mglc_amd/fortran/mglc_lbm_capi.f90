@@ -79,6 +79,12 @@ module mglc_lbm_capi
             real(c_double), intent(out) :: nu_vol, nu_wall
         end function lbm_nusselt
 
+        integer(c_int) function lbm_reynolds(ctx, re) bind(C, name='lbm_reynolds')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: re
+        end function lbm_reynolds
+
         integer(c_int) function lbm_download_temperature(ctx, T) bind(C, name='lbm_download_temperature')
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx
@@ -156,6 +162,54 @@ contains
             msg(i:i) = s(i)
         enddo
     end function lbm_last_error
+
+    ! backupData (seq/bouyancy3d.F90:1011-1030): the whole device state of
+    ! ctx as one stream file
+    subroutine lbm_save_state_file(ctx, path)
+        type(c_ptr), intent(in) :: ctx
+        character(len=*), intent(in) :: path
+        integer(c_size_t) :: bytes
+        integer(c_int8_t), allocatable :: blob(:)
+        integer :: unit
+        call lbm_require(lbm_state_bytes(ctx, bytes), 'lbm_state_bytes')
+        allocate(blob(bytes))
+        call lbm_require(lbm_save_state(ctx, blob, bytes), 'lbm_save_state')
+        open(newunit=unit, file=path, access='stream', form='unformatted', status='replace', action='write')
+        write(unit) blob
+        close(unit)
+        deallocate(blob)
+        write(*,*) 'state saved to ', path
+    end subroutine lbm_save_state_file
+
+    ! loadInitField (3d/mpi_blocked/initial.F90:102,195-206): continue from a
+    ! file written by lbm_save_state_file; returns the step count it holds
+    subroutine lbm_load_state_file(ctx, path, itc)
+        type(c_ptr), intent(in) :: ctx
+        character(len=*), intent(in) :: path
+        integer, intent(out) :: itc
+        integer(c_size_t) :: bytes
+        integer(c_int8_t), allocatable :: blob(:)
+        integer :: unit, ios
+        type(lbm_info) :: info
+        call lbm_require(lbm_state_bytes(ctx, bytes), 'lbm_state_bytes')
+        allocate(blob(bytes))
+        open(newunit=unit, file=path, access='stream', form='unformatted', status='old', action='read', iostat=ios)
+        if (ios /= 0) then
+            write(*,*) 'error: cannot open restart file ', path
+            error stop 1
+        endif
+        read(unit, iostat=ios) blob
+        close(unit)
+        if (ios /= 0) then
+            write(*,*) 'error: restart file ', path, ' is shorter than this run''s state'
+            error stop 1
+        endif
+        call lbm_require(lbm_load_state(ctx, blob, bytes), 'lbm_load_state')
+        deallocate(blob)
+        call lbm_require(lbm_get_info(ctx, info), 'lbm_get_info')
+        itc = int(info%steps_done)
+        write(*,*) 'restarted from ', path, ' at itc =', itc
+    end subroutine lbm_load_state_file
 
     ! Stop with the status, in the reference's style of fatal errors
     ! (stop on failure; the reference's MPI codes call MPI_Abort).

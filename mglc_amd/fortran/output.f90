@@ -7,12 +7,18 @@
 !   MRTcavity-<itc>.bin     Fortran sequential records u, v, rho (output.f90:350-366)
 !   cavity-raw-<itc:9>.bin  raw FP64 stream rho, u, v, w, x fastest (this driver's
 !                           own, for exact comparisons)
+! and of the buoyancy driver (Buoyancy_driven_cavity/fortran/3d/mpi_blocked/
+! output.F90):
+!   buoyancyCavity-<itc>.plt  Tecplot binary, X, Y, Z, U, V, W, T   (output.F90:151-283)
+!   buoyancyCavity-<itc>.bin  Fortran sequential records u, v, w, T (output.F90:121-146)
+!   buoyancy-raw-<itc:9>.bin  raw FP64 stream rho, u, v, w, T
 ! One GPU owns the whole cavity, so the rank-0 gather of output.f90:1-133
 ! is a single download.
 module cavity_output
     implicit none
     private
     public :: output_Tecplot, getVelocity, output_binary, output_raw
+    public :: output_Tecplot_buoyancy, output_binary_buoyancy, output_raw_buoyancy
 
 contains
 
@@ -32,18 +38,20 @@ contains
         write(unit) 0
     end subroutine put_string
 
-    subroutine output_Tecplot(dir, xp, yp, zp, u, v, w, rho, nx, ny, nz, itc)
-        character(len=*), intent(in) :: dir
-        integer, intent(in) :: nx, ny, nz, itc
+    ! Tecplot binary #!TDV101, one ordered zone, point packing, float32: the
+    ! coordinates, the velocity and a seventh variable s
+    subroutine tecplot_point(path, name7, xp, yp, zp, u, v, w, s, nx, ny, nz)
+        character(len=*), intent(in) :: path, name7
+        integer, intent(in) :: nx, ny, nz
         real(8), intent(in) :: xp(0:nx+1), yp(0:ny+1), zp(0:nz+1)
-        real(8), intent(in) :: u(nx, ny, nz), v(nx, ny, nz), w(nx, ny, nz), rho(nx, ny, nz)
+        real(8), intent(in) :: u(nx, ny, nz), v(nx, ny, nz), w(nx, ny, nz), s(nx, ny, nz)
         integer :: i, j, k, unit, q
         real(4), parameter :: zone_marker = 299.0, eoh_marker = 357.0
-        character(len=8), parameter :: names(7) = ['X       ', 'Y       ', 'Z       ', 'U       ', &
-                                                  'V       ', 'W       ', 'Pressure']
+        character(len=8) :: names(7)
+        names = ['X       ', 'Y       ', 'Z       ', 'U       ', 'V       ', 'W       ', '        ']
+        names(7) = name7
 
-        open(newunit=unit, file=trim(dir)//'/MRTcavity-'//step_tag(itc)//'.plt', access='stream', &
-             form='unformatted', status='replace')
+        open(newunit=unit, file=path, access='stream', form='unformatted', status='replace')
         write(unit) '#!TDV101'
         write(unit) 1                       ! byte order marker
         call put_string(unit, 'MyFirst')    ! title
@@ -73,12 +81,32 @@ contains
             do j = 1, ny
                 do i = 1, nx
                     write(unit) real(xp(i)), real(yp(j)), real(zp(k)), real(u(i,j,k)), &
-                                real(v(i,j,k)), real(w(i,j,k)), real(rho(i,j,k) / 3.0d0)
+                                real(v(i,j,k)), real(w(i,j,k)), real(s(i,j,k))
                 enddo
             enddo
         enddo
         close(unit)
+    end subroutine tecplot_point
+
+    subroutine output_Tecplot(dir, xp, yp, zp, u, v, w, rho, nx, ny, nz, itc)
+        character(len=*), intent(in) :: dir
+        integer, intent(in) :: nx, ny, nz, itc
+        real(8), intent(in) :: xp(0:nx+1), yp(0:ny+1), zp(0:nz+1)
+        real(8), intent(in) :: u(nx, ny, nz), v(nx, ny, nz), w(nx, ny, nz), rho(nx, ny, nz)
+        call tecplot_point(trim(dir)//'/MRTcavity-'//step_tag(itc)//'.plt', 'Pressure', xp, yp, zp, u, v, w, &
+                           rho / 3.0d0, nx, ny, nz)
     end subroutine output_Tecplot
+
+    subroutine output_Tecplot_buoyancy(dir, xp, yp, zp, u, v, w, T, nx, ny, nz, itc)
+        character(len=*), intent(in) :: dir
+        integer, intent(in) :: nx, ny, nz, itc
+        real(8), intent(in) :: xp(0:nx+1), yp(0:ny+1), zp(0:nz+1)
+        real(8), intent(in) :: u(nx, ny, nz), v(nx, ny, nz), w(nx, ny, nz), T(nx, ny, nz)
+        character(len=32) :: tag
+        write(tag, *) itc
+        call tecplot_point(trim(dir)//'/buoyancyCavity-'//trim(adjustl(tag))//'.plt', 'T', xp, yp, zp, u, v, w, &
+                           T, nx, ny, nz)
+    end subroutine output_Tecplot_buoyancy
 
     subroutine getVelocity(dir, xp, zp, u, w, nx, ny, nz, U0, itc)
         character(len=*), intent(in) :: dir
@@ -126,5 +154,32 @@ contains
         write(unit) rho, u, v, w
         close(unit)
     end subroutine output_raw
+
+    subroutine output_binary_buoyancy(dir, u, v, w, T, nx, ny, nz, itc)
+        character(len=*), intent(in) :: dir
+        integer, intent(in) :: nx, ny, nz, itc
+        real(8), intent(in) :: u(nx, ny, nz), v(nx, ny, nz), w(nx, ny, nz), T(nx, ny, nz)
+        integer :: unit
+        character(len=32) :: tag
+        write(tag, *) itc
+        open(newunit=unit, file=trim(dir)//'/buoyancyCavity-'//trim(adjustl(tag))//'.bin', form='unformatted', &
+             access='sequential', status='replace')
+        write(unit) u
+        write(unit) v
+        write(unit) w
+        write(unit) T
+        close(unit)
+    end subroutine output_binary_buoyancy
+
+    subroutine output_raw_buoyancy(dir, rho, u, v, w, T, nx, ny, nz, itc)
+        character(len=*), intent(in) :: dir
+        integer, intent(in) :: nx, ny, nz, itc
+        real(8), intent(in) :: rho(nx, ny, nz), u(nx, ny, nz), v(nx, ny, nz), w(nx, ny, nz), T(nx, ny, nz)
+        integer :: unit
+        open(newunit=unit, file=trim(dir)//'/buoyancy-raw-'//step_tag(itc)//'.bin', form='unformatted', &
+             access='stream', status='replace')
+        write(unit) rho, u, v, w, T
+        close(unit)
+    end subroutine output_raw_buoyancy
 
 end module cavity_output

@@ -196,7 +196,7 @@ class CavityConfig:
 def splitmix_uniform(shape, seed: int, amp: float = 1e-4):
     """Uniform numbers in [-amp, amp) from SplitMix64 of the linear cell index
     (x fastest) plus `seed`: the seeded noise of the buoyancy case's initial
-    temperature, reproducible on any host (mglc_amd/fortran/buoyancy.f90 uses
+    temperature, reproducible on any host (mglc_amd/fortran/buoyancy_data.f90 uses
     the same generator)."""
     n = int(np.prod(shape))
     with np.errstate(over="ignore"):
