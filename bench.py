@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--halo-loopback", type=int, default=1,
                     help="cavity: 1 = pack/unpack the six faces and twelve edges every step, looped back")
+    ap.add_argument("--attainable", type=int, default=1,
+                    help="cavity / batch: also time the step's traffic with no arithmetic (lbm_bandwidth_probe)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -109,6 +111,19 @@ def roofline(bytes_per_launch, ms_per_launch, traffic=None):
     achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
+
+
+def attainable(storage, n, achieved):
+    """What the memory system delivers for this traffic with no arithmetic,
+    measured now on this device: a one-stream copy of the same bytes and the
+    D3Q19 pull pattern of an n^3 lattice (19 upstream loads, 19 stores per
+    cell) in the library's layout; the kernel's achieved rate as a fraction
+    of the latter."""
+    from mglc_amd.lbm import bandwidth_probe
+    copy = bandwidth_probe(0, storage, n)
+    pull = bandwidth_probe(1, storage, n)
+    return {"copy_one_stream_GBs": round(copy, 1), "pull19_no_arithmetic_GBs": round(pull, 1),
+            "kernel_frac_of_pull19": round(achieved / pull, 4), "probe_n": n, "storage": storage}
 
 
 def line_head(args, value, ms_step, dtype, data, config):
@@ -204,6 +219,10 @@ def bench_cavity(args, stream):
                                 traffic_from_profiles(f"{args.n}^3 {args.collision} f64"))
     line["mass_drift_f64"] = abs(float(primary["fields"][0].sum()) - m0) / m0
     line["variants"] = variants
+    if args.attainable:
+        line["roofline"]["attainable"] = attainable("f64", args.n, line["roofline"]["achieved"])
+        for st, v in variants.items():
+            v["attainable"] = attainable(st, args.n, v["kernel_hbm_frac"] * HBM_PEAK_GBS)
     if args.cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_cavity(args)
     return line
@@ -241,6 +260,12 @@ def bench_batch(args, stream_unused):
     # kernel time per launch: the graph holds one step kernel and one 1-thread
     # step-counter kernel per step
     line["roofline"] = roofline(info.bytes_per_lup * cells, ms / args.steps, traffic_from_profiles("batch64 f32"))
+    if args.attainable:
+        # probed on one cube of about the batch's total cell count (the
+        # instances together stream as much data), edge a multiple of 64 so
+        # that whole waves cover its rows
+        edge = max(64, int(cells ** (1.0 / 3.0)) // 64 * 64)
+        line["roofline"]["attainable"] = attainable("f32", edge, line["roofline"]["achieved"])
     line["one_at_a_time"] = {"mlups": round(mlups_each, 1), "ms_per_step": round(ms_each / each_steps, 4),
                              "launches_per_step": len(cfgs), "speedup_batched": round(mlups / mlups_each, 3)}
     if args.cpu_baseline:

@@ -35,7 +35,7 @@ EXPORTED_SYMBOLS = (
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_initial", "lbm_batch_step", "lbm_batch_step_each",
     "lbm_batch_autotune", "lbm_batch_set_launch", "lbm_batch_synchronize", "lbm_batch_download_macro",
     "lbm_batch_get_info", "lbm_particles_init", "lbm_particles_get", "lbm_particles_map",
-    "lbm_state_bytes", "lbm_save_state", "lbm_load_state", "lbm_reynolds",
+    "lbm_state_bytes", "lbm_save_state", "lbm_load_state", "lbm_reynolds", "lbm_bandwidth_probe",
 )
 
 FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = range(6)
@@ -138,6 +138,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lbm_state_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
     lib.lbm_save_state.argtypes = [vp, vp, ctypes.c_size_t]
     lib.lbm_load_state.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.lbm_bandwidth_probe.argtypes = [ip, ip, ip, ip, dp]
     lib.lbm_status_string.argtypes = [ip]
     lib.lbm_status_string.restype = ctypes.c_char_p
     lib.lbm_last_error.restype = ctypes.c_char_p
@@ -146,6 +147,19 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
             getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
+
+
+def bandwidth_probe(pattern: int, storage: str = "f64", n: int = 256, reps: int = 20) -> float:
+    """GB/s (read + write) the device delivers for the step's traffic with no
+    arithmetic: pattern 0 = one-stream copy, 1 = the D3Q19 pull of an n^3
+    lattice in the library's layout (lbm_bandwidth_probe)."""
+    lib = load_library()
+    g = ctypes.c_double()
+    st = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16}[storage]
+    rc = lib.lbm_bandwidth_probe(int(pattern), st, int(n), int(reps), ctypes.byref(g))
+    if rc != 0:
+        raise LbmError(rc, "lbm_bandwidth_probe", lib.lbm_last_error().decode())
+    return g.value
 
 
 def _dp(a):

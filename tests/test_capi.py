@@ -78,3 +78,14 @@ def test_fortran_driver_binds_only_declared_symbols():
                          check=True).stdout
     used = sorted({l.split()[-1] for l in out.splitlines() if " lbm_" in l})
     assert used and set(used) <= set(declared_functions())
+
+
+def test_bandwidth_probe_rejects_bad_arguments(hip_lib):
+    """lbm_bandwidth_probe validates before any device call."""
+    g = ctypes.c_double()
+    assert hip_lib.lbm_bandwidth_probe(2, 0, 64, 1, ctypes.byref(g)) == 1
+    assert hip_lib.lbm_bandwidth_probe(0, 3, 64, 1, ctypes.byref(g)) == 1
+    assert hip_lib.lbm_bandwidth_probe(1, 0, 8, 1, ctypes.byref(g)) == 1
+    assert hip_lib.lbm_bandwidth_probe(1, 0, 64, 0, ctypes.byref(g)) == 1
+    assert hip_lib.lbm_bandwidth_probe(1, 0, 64, 1, None) == 1
+    assert b"lbm_bandwidth_probe" in hip_lib.lbm_last_error()
