@@ -86,7 +86,13 @@ typedef struct lbm_params {
     int thermal_axis;     /* thermal: axis of the walls held at t_hot (low side) and t_cold
                              (high side): 1 = y (LeftRightWallsConstT), 2 = z
                              (TopBottomPlatesConstT), 0 = x, -1 = none; other walls adiabatic */
-    int reserved_i[3];
+    int halo_overlap;     /* 1: split every step into the boundary shell (cells on halo
+                             faces) and the interior, as the reference's non-blocking code
+                             does (lid3_mpi_nonblock.f90:1108-1231); with halo_loopback the
+                             exchange then runs on a second stream while the interior is
+                             updated.  Halo faces must be y and z faces only (x, the
+                             contiguous axis, stays whole); isothermal contexts only */
+    int reserved_i[2];
     double tau;           /* tauf: commondata.f90:9 tauf = 3 U0 N / Re + 0.5
                              (thermal: module.F90:19 tauf = 0.5 + Ma N sqrt(3 Pr / Ra)) */
     double u_lid;         /* U0: lid velocity along +x on the top wall k = nz (0 when thermal) */
@@ -140,6 +146,19 @@ int lbm_load_fields(lbm_ctx *ctx, const double *rho, const double *u, const doub
  * (lbm_halo_pack / transport / lbm_halo_unpack), so nsteps must be 1. */
 int lbm_step(lbm_ctx *ctx, int nsteps);
 
+/* One step in two halves, for a caller that overlaps its own exchange with the
+ * interior update (lid3_mpi_nonblock.f90:1108-1231: boundary collision, then
+ * Isend/Irecv, then interior collision, then Waitall).  Requires halo_overlap.
+ *   lbm_step_begin: update the boundary shell (every cell on a halo face) of
+ *                   the next step; lbm_halo_pack called now packs that shell;
+ *   lbm_step_end:   update the interior and complete the step (macro fields
+ *                   written, as lbm_step(ctx, 1)).
+ * The usual sequence per step: lbm_halo_unpack(previous receive), begin, pack,
+ * start the transport on another stream, end, wait for the transport.  Results
+ * equal lbm_step(ctx, 1) bit for bit. */
+int lbm_step_begin(lbm_ctx *ctx);
+int lbm_step_end(lbm_ctx *ctx);
+
 /* Reference check (check.f90:9-34): errorU = sqrt(sum (u-up)^2 + (v-vp)^2) /
  * sqrt(sum u^2 + v^2 + w^2) over the last macro fields, then up = u, vp = v.
  * FP64 device reduction.  Returns LBM_ERR_NAN (with *error_u set) when the
@@ -180,7 +199,9 @@ int lbm_halo_buffer_bytes(const lbm_ctx *ctx, size_t *bytes);
 /* Pack every outgoing region of the current state into the device buffer
  * dev_send (lbm_halo_buffer_bytes long): the post-collision populations that
  * leave through each face and edge, as the exchange of ex_sendrecv.f90 sends
- * them.  Call after lbm_initial / lbm_step and before the next lbm_step. */
+ * them.  Call after lbm_initial / lbm_step and before the next lbm_step;
+ * between lbm_step_begin and lbm_step_end it packs the shell of the step in
+ * progress (regions of faces in halo_faces and edges between two of them). */
 int lbm_halo_pack(lbm_ctx *ctx, void *dev_send);
 
 /* Take the incoming regions from the device buffer dev_recv (same layout;

@@ -47,6 +47,50 @@ __global__ __launch_bounds__(256) void k_cavity_step(const StepArgs a) {
     cavity_cell<S, COLL, MACRO, HALO>(a, i, j, k);
 }
 
+// The split step (halo_overlap): the boundary shell -- every cell on a halo
+// face, the cells whose populations the neighbours need next -- and the
+// interior, the rest.  Halo faces are y and z faces only, so the shell is
+// whole x rows: first the z planes of the halo z faces (every j), then the
+// rows j of the halo y faces on the remaining k.  Each cell belongs to
+// exactly one of the two kernels and is updated by the same cavity_cell as
+// the fused kernel, so the split step equals the fused one bit for bit.
+struct ShellRows {
+    int nzf, kf[2];  // halo z faces: their planes k
+    int njf, jf[2];  // halo y faces: their rows j
+    int k0, nzr;     // the k range [k0, k0 + nzr) not on a halo z face
+    int rows;        // nzf * ny + njf * nzr
+};
+
+template <class S, int COLL, bool MACRO>
+__global__ __launch_bounds__(256) void k_cavity_shell(const StepArgs a, const ShellRows r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nx) return;
+    const int row = blockIdx.y;
+    int j, k;
+    if (row < r.nzf * a.ny) {
+        const int p = row / a.ny;
+        j = row - p * a.ny;
+        k = r.kf[p];
+    } else {
+        const int t = row - r.nzf * a.ny;
+        const int q = t / r.nzr;
+        j = r.jf[q];
+        k = r.k0 + (t - q * r.nzr);
+    }
+    cavity_cell<S, COLL, MACRO, true>(a, i, j, k);
+}
+
+// Interior: j in [j0, j1), k in [k0, k0 + gridDim.z); no pull from these
+// cells crosses a halo face, so the halo path is compiled out.
+template <class S, int COLL, bool MACRO>
+__global__ __launch_bounds__(256) void k_cavity_interior(const StepArgs a, int j0, int j1, int k0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = j0 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = k0 + blockIdx.z;
+    if (i >= a.nx || j >= j1) return;
+    cavity_cell<S, COLL, MACRO, false>(a, i, j, k);
+}
+
 struct LeavingTable {
     int pop[NFACE][5];
 };
@@ -62,8 +106,9 @@ constexpr LeavingTable LEAVING = make_leaving();
 // grid.y = region; grid.x covers the largest region.
 template <class T>
 __global__ __launch_bounds__(256) void k_halo_pack(const T* __restrict__ src, T* __restrict__ buf,
-                                                   const StepArgs a) {
+                                                   const StepArgs a, unsigned mask) {
     const int r = blockIdx.y;
+    if (!((mask >> r) & 1u)) return;
     const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     int i, j, k, al;
     if (r < NFACE) {
@@ -97,8 +142,9 @@ __global__ __launch_bounds__(256) void k_halo_pack(const T* __restrict__ src, T*
 // one GPU.
 template <class T>
 __global__ __launch_bounds__(256) void k_halo_unpack(const T* __restrict__ recv, T* __restrict__ halo,
-                                                     const StepArgs a, int loopback) {
+                                                     const StepArgs a, int loopback, unsigned mask) {
     const int r = blockIdx.y;
+    if (!((mask >> r) & 1u)) return;
     const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (e >= a.hcount[r]) return;
     const int from = (loopback && r < NFACE) ? (r ^ 1) : r;
@@ -251,6 +297,8 @@ __global__ void k_fill_layers(double* __restrict__ T, int nx, int ny, int nz, in
 
 using StepFn = void (*)(const StepArgs);
 using InitFn = void (*)(const StepArgs, const double*, const double*, const double*, const double*);
+using ShellFn = void (*)(const StepArgs, const ShellRows);
+using InteriorFn = void (*)(const StepArgs, int, int, int);
 
 template <class S>
 struct Table {
@@ -260,6 +308,12 @@ struct Table {
         {{k_cavity_step<S, COLL_BGK, false, false>, k_cavity_step<S, COLL_BGK, false, true>},
          {k_cavity_step<S, COLL_BGK, true, false>, k_cavity_step<S, COLL_BGK, true, true>}}};
     static constexpr InitFn init[2] = {k_cavity_init<S, COLL_MRT>, k_cavity_init<S, COLL_BGK>};
+    static constexpr ShellFn shell[2][2] = {
+        {k_cavity_shell<S, COLL_MRT, false>, k_cavity_shell<S, COLL_MRT, true>},
+        {k_cavity_shell<S, COLL_BGK, false>, k_cavity_shell<S, COLL_BGK, true>}};
+    static constexpr InteriorFn interior[2][2] = {
+        {k_cavity_interior<S, COLL_MRT, false>, k_cavity_interior<S, COLL_MRT, true>},
+        {k_cavity_interior<S, COLL_BGK, false>, k_cavity_interior<S, COLL_BGK, true>}};
 };
 
 }  // namespace
@@ -316,42 +370,155 @@ long long halo_max_region(const lbm_ctx* c) {
     return m;
 }
 
+constexpr unsigned ALL_REGIONS = (1u << NREGION) - 1u;
+
 // Launch the pack (dst = send buffer) or unpack (src = receive buffer) kernel
-// for the context's storage type.
-int launch_pack(lbm_ctx* c, const void* state, void* send) {
+// for the context's storage type, for the regions in `mask`, on `stream`.
+int launch_pack(lbm_ctx* c, const void* state, void* send, unsigned mask = ALL_REGIONS,
+                hipStream_t stream = nullptr) {
     const StepArgs a = make_args(c);
     const dim3 grid((unsigned)((halo_max_region(c) + 255) / 256), NREGION);
+    if (!stream) stream = c->stream;
     switch (c->p.storage) {
     case LBM_STORE_F32:
-        hipLaunchKernelGGL(k_halo_pack<float>, grid, dim3(256), 0, c->stream, (const float*)state, (float*)send, a);
+        hipLaunchKernelGGL(k_halo_pack<float>, grid, dim3(256), 0, stream, (const float*)state, (float*)send, a,
+                           mask);
         break;
     case LBM_STORE_F16:
-        hipLaunchKernelGGL(k_halo_pack<_Float16>, grid, dim3(256), 0, c->stream, (const _Float16*)state,
-                           (_Float16*)send, a);
+        hipLaunchKernelGGL(k_halo_pack<_Float16>, grid, dim3(256), 0, stream, (const _Float16*)state,
+                           (_Float16*)send, a, mask);
         break;
     default:
-        hipLaunchKernelGGL(k_halo_pack<double>, grid, dim3(256), 0, c->stream, (const double*)state, (double*)send, a);
+        hipLaunchKernelGGL(k_halo_pack<double>, grid, dim3(256), 0, stream, (const double*)state, (double*)send, a,
+                           mask);
     }
     return lbm_check_launch();
 }
 
-int launch_unpack(lbm_ctx* c, const void* recv, int loopback) {
+int launch_unpack(lbm_ctx* c, const void* recv, int loopback, unsigned mask = ALL_REGIONS,
+                  hipStream_t stream = nullptr) {
     const StepArgs a = make_args(c);
     const dim3 grid((unsigned)((halo_max_region(c) + 255) / 256), NREGION);
+    if (!stream) stream = c->stream;
     switch (c->p.storage) {
     case LBM_STORE_F32:
-        hipLaunchKernelGGL(k_halo_unpack<float>, grid, dim3(256), 0, c->stream, (const float*)recv, (float*)c->hin,
-                           a, loopback);
+        hipLaunchKernelGGL(k_halo_unpack<float>, grid, dim3(256), 0, stream, (const float*)recv, (float*)c->hin,
+                           a, loopback, mask);
         break;
     case LBM_STORE_F16:
-        hipLaunchKernelGGL(k_halo_unpack<_Float16>, grid, dim3(256), 0, c->stream, (const _Float16*)recv,
-                           (_Float16*)c->hin, a, loopback);
+        hipLaunchKernelGGL(k_halo_unpack<_Float16>, grid, dim3(256), 0, stream, (const _Float16*)recv,
+                           (_Float16*)c->hin, a, loopback, mask);
         break;
     default:
-        hipLaunchKernelGGL(k_halo_unpack<double>, grid, dim3(256), 0, c->stream, (const double*)recv, (double*)c->hin,
-                           a, loopback);
+        hipLaunchKernelGGL(k_halo_unpack<double>, grid, dim3(256), 0, stream, (const double*)recv, (double*)c->hin,
+                           a, loopback, mask);
     }
     return lbm_check_launch();
+}
+
+// Regions whose cells all lie on halo faces: faces in halo_faces, and edges
+// whose two faces (the ones the diagonal population leaves through) both are.
+// With a face mask symmetric per axis these are also the regions read.
+unsigned shell_regions(const lbm_ctx* c) {
+    const int m = c->p.halo_faces;
+    unsigned mask = 0;
+    for (int f = 0; f < NFACE; ++f)
+        if ((m >> f) & 1) mask |= 1u << f;
+    for (int al = 7; al < Q; ++al) {
+        bool all = true;
+        for (int axis = 0; axis < 3; ++axis) {
+            const int e = ecomp(al, axis);
+            if (e != 0 && !((m >> (2 * axis + (e > 0 ? 1 : 0))) & 1)) all = false;
+        }
+        if (all) mask |= 1u << edge_region(al);
+    }
+    return mask;
+}
+
+ShellRows shell_rows(const lbm_ctx* c) {
+    const int m = c->p.halo_faces;
+    ShellRows r{};
+    if ((m >> LBM_FACE_ZM) & 1) r.kf[r.nzf++] = 0;
+    if ((m >> LBM_FACE_ZP) & 1) r.kf[r.nzf++] = c->p.nz - 1;
+    if ((m >> LBM_FACE_YM) & 1) r.jf[r.njf++] = 0;
+    if ((m >> LBM_FACE_YP) & 1) r.jf[r.njf++] = c->p.ny - 1;
+    r.k0 = ((m >> LBM_FACE_ZM) & 1) ? 1 : 0;
+    r.nzr = c->p.nz - r.nzf;
+    r.rows = r.nzf * c->p.ny + r.njf * r.nzr;
+    return r;
+}
+
+// Launch the two halves of one split step for args `a`.
+int launch_shell(lbm_ctx* c, const StepArgs& a, bool macro) {
+    const ShellRows r = shell_rows(c);
+    if (r.rows == 0) return LBM_OK;
+    const int coll = c->p.collision == LBM_BGK ? 1 : 0;
+    ShellFn fn;
+    switch (c->p.storage) {
+    case LBM_STORE_F32: fn = Table<StoreF32>::shell[coll][macro]; break;
+    case LBM_STORE_F16: fn = Table<StoreF16>::shell[coll][macro]; break;
+    default: fn = Table<StoreF64>::shell[coll][macro];
+    }
+    const int bx = c->block.x;
+    hipLaunchKernelGGL(fn, dim3((c->p.nx + bx - 1) / bx, r.rows), dim3(bx), 0, c->stream, a, r);
+    return lbm_check_launch();
+}
+
+int launch_interior(lbm_ctx* c, const StepArgs& a, bool macro) {
+    const int m = c->p.halo_faces;
+    const int j0 = ((m >> LBM_FACE_YM) & 1) ? 1 : 0, j1 = c->p.ny - (((m >> LBM_FACE_YP) & 1) ? 1 : 0);
+    const int k0 = ((m >> LBM_FACE_ZM) & 1) ? 1 : 0, k1 = c->p.nz - (((m >> LBM_FACE_ZP) & 1) ? 1 : 0);
+    if (j1 <= j0 || k1 <= k0) return LBM_OK;
+    const int coll = c->p.collision == LBM_BGK ? 1 : 0;
+    InteriorFn fn;
+    switch (c->p.storage) {
+    case LBM_STORE_F32: fn = Table<StoreF32>::interior[coll][macro]; break;
+    case LBM_STORE_F16: fn = Table<StoreF16>::interior[coll][macro]; break;
+    default: fn = Table<StoreF64>::interior[coll][macro];
+    }
+    const dim3 grid((c->p.nx + c->block.x - 1) / c->block.x, (j1 - j0 + c->block.y - 1) / c->block.y, k1 - k0);
+    hipLaunchKernelGGL(fn, grid, c->block, 0, c->stream, a, j0, j1, k0);
+    return lbm_check_launch();
+}
+
+// nsteps split steps.  With the loopback the exchange of each step's shell
+// runs on the second stream while the interior is updated: the shell's
+// completion releases the pack, and the next step's shell waits for the
+// unpack.  The halo of the current state is exchanged first, on the main
+// stream; the last step's shell is not exchanged (the next call starts with
+// that exchange).
+int step_split(lbm_ctx* c, int nsteps) {
+    const bool loop = c->p.halo_loopback != 0;
+    const unsigned mask = shell_regions(c);
+    StepArgs a = make_args(c);
+    int rc = LBM_OK;
+    if (loop && nsteps > 0) {
+        rc = launch_pack(c, c->buf[c->cur], c->hsend, mask);
+        if (!rc) rc = launch_unpack(c, c->hsend, 1, mask);
+        if (rc) return rc;
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        const bool macro = s == nsteps - 1;
+        const bool exchange = loop && s < nsteps - 1;
+        a.src = c->buf[c->cur];
+        a.dst = c->buf[1 - c->cur];
+        a.seed = (unsigned)(c->itc + s + 1);
+        rc = launch_shell(c, a, macro);
+        if (rc) return rc;
+        if (exchange) {
+            HIP_TRY(hipEventRecord(c->ev_shell, c->stream));
+            HIP_TRY(hipStreamWaitEvent(c->aux, c->ev_shell, 0));
+            rc = launch_pack(c, a.dst, c->hsend, mask, c->aux);
+            if (!rc) rc = launch_unpack(c, c->hsend, 1, mask, c->aux);
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(c->ev_halo, c->aux));
+        }
+        rc = launch_interior(c, a, macro);
+        if (rc) return rc;
+        c->cur = 1 - c->cur;
+        if (exchange) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_halo, 0));
+    }
+    return LBM_OK;
 }
 
 InitFn init_fn(const lbm_ctx* c) {
@@ -374,6 +541,9 @@ void release(lbm_ctx* c) {
     if (c->hsend) (void)hipFree(c->hsend);
     thermal_release(c);
     particles_release(c);
+    if (c->ev_halo) (void)hipEventDestroy(c->ev_halo);
+    if (c->ev_shell) (void)hipEventDestroy(c->ev_shell);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
 }
 
 }  // namespace
@@ -466,6 +636,24 @@ int lbm_create(const lbm_params* p, lbm_ctx** out) {
 
     if (p->halo_faces < 0 || p->halo_faces > 63)
         return lbm_fail(LBM_ERR_ARG, "lbm_create: halo_faces 0x%x is not a 6-bit face mask", p->halo_faces);
+    if (p->halo_overlap != 0 && p->halo_overlap != 1)
+        return lbm_fail(LBM_ERR_ARG, "lbm_create: halo_overlap = %d", p->halo_overlap);
+    if (p->halo_overlap) {
+        if (p->thermal)
+            return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_create: the thermal path has no split step");
+        if (p->halo_faces & ((1 << LBM_FACE_XM) | (1 << LBM_FACE_XP)))
+            return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_create: the split step keeps x whole; decompose along y "
+                                                 "and z (halo_faces 0x%x)", p->halo_faces);
+        if (p->halo_faces == 0)
+            return lbm_fail(LBM_ERR_ARG, "lbm_create: halo_overlap without halo faces has nothing to overlap");
+        const int m = p->halo_faces;
+        if (p->halo_loopback && (((m >> 2) & 1) != ((m >> 3) & 1) || ((m >> 4) & 1) != ((m >> 5) & 1)))
+            return lbm_fail(LBM_ERR_ARG, "lbm_create: a looped-back split step needs both faces of each halo axis "
+                                         "(halo_faces 0x%x)", m);
+        const long long rows = 2LL * p->ny + 2LL * p->nz;
+        if (rows > 65535)
+            return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_create: %lld shell rows exceed the launch grid", rows);
+    }
 
     lbm_ctx* c = new (std::nothrow) lbm_ctx();
     if (!c) return lbm_fail(LBM_ERR_NOMEM, "lbm_create: host allocation failed");
@@ -536,6 +724,15 @@ int lbm_create(const lbm_params* p, lbm_ctx** out) {
             return rc == LBM_ERR_HIP ? LBM_ERR_NOMEM : rc;
         }
     }
+    if (p->halo_overlap &&
+        (hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&c->ev_shell, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&c->ev_halo, hipEventDisableTiming) != hipSuccess)) {
+        (void)hipGetLastError();
+        release(c);
+        delete c;
+        return lbm_fail(LBM_ERR_HIP, "lbm_create: creating the exchange stream failed");
+    }
     // incoming halos start as zeros until the first exchange
     if (hipMemset(c->hin, 0, c->elem * c->htotal) != hipSuccess) {
         release(c);
@@ -587,6 +784,7 @@ static int load_device_fields(lbm_ctx* c, const double* d_rho, const double* d_u
 
 int lbm_initial(lbm_ctx* c) {
     if (!c) return lbm_fail(LBM_ERR_ARG, "lbm_initial: null context");
+    c->in_step = false;  // a fresh start abandons a step in progress
     particles_release(c);  // a fresh start detaches any spheres
     // initial fields into the macro arrays, then equilibrium + first collision
     // from them.  The init kernel reads and rewrites the same macro values
@@ -614,6 +812,7 @@ int lbm_initial(lbm_ctx* c) {
 int lbm_load_fields(lbm_ctx* c, const double* rho, const double* u, const double* v, const double* w,
                     const double* T) {
     if (!c || !rho || !u || !v || !w) return lbm_fail(LBM_ERR_ARG, "lbm_load_fields: null argument");
+    c->in_step = false;
     particles_release(c);
     const size_t bytes = sizeof(double) * c->n;
     if (c->p.thermal) {
@@ -636,6 +835,7 @@ int lbm_step(lbm_ctx* c, int nsteps) {
     if (!c) return lbm_fail(LBM_ERR_ARG, "lbm_step: null context");
     if (nsteps < 0) return lbm_fail(LBM_ERR_ARG, "lbm_step: nsteps = %d", nsteps);
     if (!c->initialised) return lbm_fail(LBM_ERR_STATE, "lbm_step before lbm_initial / lbm_load_fields");
+    if (c->in_step) return lbm_fail(LBM_ERR_STATE, "lbm_step between lbm_step_begin and lbm_step_end");
     if (c->p.thermal) {
         int rc = thermal_step(c, nsteps);
         c->itc += nsteps;
@@ -649,6 +849,12 @@ int lbm_step(lbm_ctx* c, int nsteps) {
     if (c->p.halo_faces != 0 && !c->p.halo_loopback && nsteps > 1)
         return lbm_fail(LBM_ERR_STATE, "lbm_step: with halo faces and no loopback the halo must be exchanged "
                                    "between steps; call lbm_step(ctx, 1)");
+    if (c->p.halo_overlap) {
+        int rc = step_split(c, nsteps);
+        if (rc) return rc;
+        c->itc += nsteps;
+        return lbm_check_launch();
+    }
     StepArgs a = make_args(c);
     const dim3 grid = grid_of(c);
     for (int s = 0; s < nsteps; ++s) {
@@ -670,6 +876,7 @@ int lbm_step(lbm_ctx* c, int nsteps) {
 int lbm_check_sums(lbm_ctx* c, double* e1, double* e2) {
     if (!c || !e1 || !e2) return lbm_fail(LBM_ERR_ARG, "lbm_check_sums: null argument");
     if (!c->initialised) return lbm_fail(LBM_ERR_STATE, "lbm_check before lbm_initial");
+    if (c->in_step) return lbm_fail(LBM_ERR_STATE, "lbm_check between lbm_step_begin and lbm_step_end");
     if (c->p.thermal) {
         double s4[4];
         int rc = thermal_check(c, s4);
@@ -757,7 +964,38 @@ int lbm_halo_buffer_bytes(const lbm_ctx* c, size_t* bytes) {
 int lbm_halo_pack(lbm_ctx* c, void* dev_send) {
     if (!c || !dev_send) return lbm_fail(LBM_ERR_ARG, "lbm_halo_pack: null argument");
     if (!c->initialised) return lbm_fail(LBM_ERR_STATE, "lbm_halo_pack before lbm_initial");
-    return launch_pack(c, c->buf[c->cur], dev_send);
+    // between lbm_step_begin and lbm_step_end: the shell of the step in progress
+    return launch_pack(c, c->buf[c->in_step ? 1 - c->cur : c->cur], dev_send);
+}
+
+int lbm_step_begin(lbm_ctx* c) {
+    if (!c) return lbm_fail(LBM_ERR_ARG, "lbm_step_begin: null context");
+    if (!c->p.halo_overlap) return lbm_fail(LBM_ERR_STATE, "lbm_step_begin: the context was created without halo_overlap");
+    if (!c->initialised) return lbm_fail(LBM_ERR_STATE, "lbm_step_begin before lbm_initial / lbm_load_fields");
+    if (c->in_step) return lbm_fail(LBM_ERR_STATE, "lbm_step_begin: a step is already in progress");
+    StepArgs a = make_args(c);
+    a.src = c->buf[c->cur];
+    a.dst = c->buf[1 - c->cur];
+    a.seed = (unsigned)(c->itc + 1);
+    int rc = launch_shell(c, a, true);
+    if (rc) return rc;
+    c->in_step = true;
+    return LBM_OK;
+}
+
+int lbm_step_end(lbm_ctx* c) {
+    if (!c) return lbm_fail(LBM_ERR_ARG, "lbm_step_end: null context");
+    if (!c->in_step) return lbm_fail(LBM_ERR_STATE, "lbm_step_end without lbm_step_begin");
+    StepArgs a = make_args(c);
+    a.src = c->buf[c->cur];
+    a.dst = c->buf[1 - c->cur];
+    a.seed = (unsigned)(c->itc + 1);
+    int rc = launch_interior(c, a, true);
+    if (rc) return rc;
+    c->cur = 1 - c->cur;
+    c->itc += 1;
+    c->in_step = false;
+    return LBM_OK;
 }
 
 int lbm_halo_unpack(lbm_ctx* c, const void* dev_recv) {
@@ -768,6 +1006,7 @@ int lbm_halo_unpack(lbm_ctx* c, const void* dev_recv) {
 int lbm_download_macro(lbm_ctx* c, double* rho, double* u, double* v, double* w) {
     if (!c) return lbm_fail(LBM_ERR_ARG, "lbm_download_macro: null context");
     if (!c->initialised) return lbm_fail(LBM_ERR_STATE, "lbm_download_macro before lbm_initial");
+    if (c->in_step) return lbm_fail(LBM_ERR_STATE, "lbm_download_macro between lbm_step_begin and lbm_step_end");
     double* dst[4] = {rho, u, v, w};
     const size_t bytes = sizeof(double) * c->n;
     for (int q = 0; q < 4; ++q)

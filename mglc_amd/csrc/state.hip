@@ -72,6 +72,7 @@ int usable(lbm_ctx* c, const char* who) {
     if (!c) return lbm_fail(LBM_ERR_ARG, "%s: null context", who);
     if (c->part)
         return lbm_fail(LBM_ERR_UNSUPPORTED, "%s: contexts with attached spheres are not checkpointed", who);
+    if (c->in_step) return lbm_fail(LBM_ERR_STATE, "%s between lbm_step_begin and lbm_step_end", who);
     return LBM_OK;
 }
 

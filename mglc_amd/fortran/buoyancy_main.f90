@@ -74,6 +74,7 @@ program buoyancy_main
     p%halo_faces = 0
     p%halo_loopback = 0
     p%thermal_axis = heated_axis
+    p%halo_overlap = 0
     p%reserved_i = 0
     p%tau = tauf
     p%u_lid = 0.0d0

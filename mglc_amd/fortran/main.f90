@@ -53,6 +53,7 @@ program main
     p%halo_faces = 0
     p%halo_loopback = merge(1, 0, halo_loopback)
     p%thermal_axis = -1
+    p%halo_overlap = 0
     p%reserved_i = 0
     p%tau = tauf
     p%u_lid = U0

@@ -28,7 +28,8 @@ module mglc_lbm_capi
         integer(c_int) :: halo_faces
         integer(c_int) :: halo_loopback
         integer(c_int) :: thermal_axis
-        integer(c_int) :: reserved_i(3)
+        integer(c_int) :: halo_overlap
+        integer(c_int) :: reserved_i(2)
         real(c_double) :: tau
         real(c_double) :: u_lid
         real(c_double) :: rho0

@@ -36,6 +36,7 @@ EXPORTED_SYMBOLS = (
     "lbm_batch_autotune", "lbm_batch_set_launch", "lbm_batch_synchronize", "lbm_batch_download_macro",
     "lbm_batch_get_info", "lbm_particles_init", "lbm_particles_get", "lbm_particles_map",
     "lbm_state_bytes", "lbm_save_state", "lbm_load_state", "lbm_reynolds", "lbm_bandwidth_probe",
+    "lbm_step_begin", "lbm_step_end",
 )
 
 FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = range(6)
@@ -49,7 +50,7 @@ class LbmParams(ctypes.Structure):
         ("f16_scale_log2", ctypes.c_int), ("thermal", ctypes.c_int),
         ("block_x", ctypes.c_int), ("block_y", ctypes.c_int),
         ("halo_faces", ctypes.c_int), ("halo_loopback", ctypes.c_int), ("thermal_axis", ctypes.c_int),
-        ("reserved_i", ctypes.c_int * 3),
+        ("halo_overlap", ctypes.c_int), ("reserved_i", ctypes.c_int * 2),
         ("tau", ctypes.c_double), ("u_lid", ctypes.c_double), ("rho0", ctypes.c_double),
         ("para_a", ctypes.c_double), ("g_beta", ctypes.c_double), ("omega_rot", ctypes.c_double),
         ("t_hot", ctypes.c_double), ("t_cold", ctypes.c_double), ("t_ref", ctypes.c_double),
@@ -112,6 +113,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lbm_nusselt.argtypes = [vp, dp, dp]
     lib.lbm_download_temperature.argtypes = [vp, dp]
     lib.lbm_step.argtypes = [vp, ip]
+    lib.lbm_step_begin.argtypes = [vp]
+    lib.lbm_step_end.argtypes = [vp]
     lib.lbm_check.argtypes = [vp, dp]
     lib.lbm_download_macro.argtypes = [vp, dp, dp, dp, dp]
     lib.lbm_synchronize.argtypes = [vp]
@@ -188,6 +191,7 @@ class CavityConfig:
     block_y: int = 0
     halo_faces: int = 0        # bit f: face f reads a halo (FACE_XM..FACE_ZP) instead of a wall
     halo_loopback: int = 0     # 1: exchange looped back onto this subdomain before every step
+    halo_overlap: int = 0      # 1: boundary shell + interior per step, exchange overlapped with the interior
 
     def tau(self) -> float:
         if self.tauf is not None:
@@ -202,7 +206,7 @@ class CavityConfig:
         p.meq12_rho = self.meq12_rho
         p.f16_scale_log2 = self.f16_scale_log2
         p.block_x, p.block_y = self.block_x, self.block_y
-        p.halo_faces, p.halo_loopback = self.halo_faces, self.halo_loopback
+        p.halo_faces, p.halo_loopback, p.halo_overlap = self.halo_faces, self.halo_loopback, self.halo_overlap
         p.tau, p.u_lid, p.rho0 = self.tau(), self.U0, self.rho0
         return p
 
@@ -393,6 +397,14 @@ class Cavity:
 
     def step(self, nsteps: int = 1):
         self._call("lbm_step", self._ctx, int(nsteps))
+
+    def step_begin(self):
+        """Boundary shell of the next step (halo_overlap contexts); halo_pack
+        now packs it, step_end completes the step."""
+        self._call("lbm_step_begin", self._ctx)
+
+    def step_end(self):
+        self._call("lbm_step_end", self._ctx)
 
     def check(self) -> float:
         e = ctypes.c_double()

@@ -89,3 +89,25 @@ def test_bandwidth_probe_rejects_bad_arguments(hip_lib):
     assert hip_lib.lbm_bandwidth_probe(1, 0, 64, 0, ctypes.byref(g)) == 1
     assert hip_lib.lbm_bandwidth_probe(1, 0, 64, 1, None) == 1
     assert b"lbm_bandwidth_probe" in hip_lib.lbm_last_error()
+
+
+def test_create_validates_the_split_step(hip_lib):
+    """halo_overlap: y/z halo faces only, isothermal, with halo faces, and a
+    looped-back split needs both faces of each halo axis."""
+    from mglc_amd import lbm
+    ctx = ctypes.c_void_p()
+    cases = [
+        (dict(halo_faces=0x3C, halo_overlap=2), 1),
+        (dict(halo_faces=0x01, halo_overlap=1), 7),    # x face
+        (dict(halo_faces=0x00, halo_overlap=1), 1),    # nothing to overlap
+        (dict(halo_faces=0x04, halo_overlap=1, halo_loopback=1), 1),  # y- without y+
+        (dict(halo_faces=0x10, halo_overlap=1, halo_loopback=1), 1),  # z- without z+
+    ]
+    for kw, status in cases:
+        p = lbm.CavityConfig(16, 16, 16, **kw).params()
+        assert hip_lib.lbm_create(ctypes.byref(p), ctypes.byref(ctx)) == status, kw
+        assert not ctx.value
+    p = lbm.ThermalConfig(8, 8, 8).params()
+    p.halo_overlap = 1
+    assert hip_lib.lbm_create(ctypes.byref(p), ctypes.byref(ctx)) == 7
+    assert hip_lib.lbm_step_begin(None) == 1 and hip_lib.lbm_step_end(None) == 1
