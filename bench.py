@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--halo-loopback", type=int, default=1,
                     help="cavity: 1 = pack/unpack the six faces and twelve edges every step, looped back")
+    ap.add_argument("--overlap", type=int, default=1,
+                    help="cavity: also time y/z faces looped back, fused step vs split step with the exchange "
+                         "overlapped")
     ap.add_argument("--attainable", type=int, default=1,
                     help="cavity / batch: also time the step's traffic with no arithmetic (lbm_bandwidth_probe)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -169,6 +172,26 @@ def run_storage(args, storage, stream):
                 steps_total=steps_total)
 
 
+def overlap_compare(args, stream):
+    """The y and z faces looped back (a periodic y-z box: the exchange of a
+    subdomain with four neighbours, on one GPU), timed with the fused step
+    (exchange between steps) and with the split step (boundary shell first,
+    its exchange on a second stream while the interior is updated)."""
+    from mglc_amd.lbm import Cavity, CavityConfig
+
+    n = args.n
+    base = dict(Reynolds=1000.0, U0=0.1, collision=args.collision, storage="f64", halo_faces=0x3C, halo_loopback=1)
+    steps = min(args.steps, 500)
+    out = {"halo": "y and z faces looped back: 4 faces x 5 pops + 4 y-z edges per step", "storage": "f64",
+           "steps": steps}
+    for name, ov in (("fused_serial", 0), ("split_overlapped", 1)):
+        with Cavity(CavityConfig(n, n, n, halo_overlap=ov, **base), stream=stream.handle.value) as cav:
+            cav.initial()
+            ms, _ = timed(cav.step, stream, argparse.Namespace(warmup=args.warmup, steps=steps))
+            out[f"ms_per_step_{name}"] = round(ms / steps, 4)
+    return out
+
+
 def cpu_baseline_cavity(args):
     """The oracle (CPU restatement, OpenMP) on a bounded sample of the same
     workload: 256^3 cells from the reference initial state, a few steps."""
@@ -219,6 +242,10 @@ def bench_cavity(args, stream):
                                 traffic_from_profiles(f"{args.n}^3 {args.collision} f64"))
     line["mass_drift_f64"] = abs(float(primary["fields"][0].sum()) - m0) / m0
     line["variants"] = variants
+    if args.overlap:
+        ov = overlap_compare(args, stream)
+        ov["ms_per_step_no_exchange"] = round(primary["ms_kernel"] / args.steps, 4)
+        line["overlap"] = ov
     if args.attainable:
         line["roofline"]["attainable"] = attainable("f64", args.n, line["roofline"]["achieved"])
         for st, v in variants.items():

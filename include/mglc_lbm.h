@@ -89,9 +89,10 @@ typedef struct lbm_params {
     int halo_overlap;     /* 1: split every step into the boundary shell (cells on halo
                              faces) and the interior, as the reference's non-blocking code
                              does (lid3_mpi_nonblock.f90:1108-1231); with halo_loopback the
-                             exchange then runs on a second stream while the interior is
-                             updated.  Halo faces must be y and z faces only (x, the
-                             contiguous axis, stays whole); isothermal contexts only */
+                             shell kernel also writes the populations it sends straight into
+                             the next step's halo, so no pack / unpack runs between steps.
+                             Halo faces must be y and z faces only (x, the contiguous axis,
+                             stays whole); isothermal contexts only */
     int reserved_i[2];
     double tau;           /* tauf: commondata.f90:9 tauf = 3 U0 N / Re + 0.5
                              (thermal: module.F90:19 tauf = 0.5 + Ma N sqrt(3 Pr / Ra)) */

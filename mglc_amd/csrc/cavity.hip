@@ -61,7 +61,41 @@ struct ShellRows {
     int rows;        // nzf * ny + njf * nzr
 };
 
-template <class S, int COLL, bool MACRO>
+// The looped-back exchange folded into the shell: every stored population
+// that leaves through a halo face is also written where the looped-back
+// unpack puts it in the next step's halo (a.halo_out) -- the incoming region
+// of the opposite face, same slot and plane index as k_halo_pack and
+// k_halo_unpack use, and for the y-z edges the same edge region.  The step
+// then needs no pack or unpack kernel.  (Across GPUs the same stores would go
+// to the neighbour's halo over the peer link.)
+template <class S>
+struct LoopbackHalo {
+    const StepArgs& a;
+    int i, j, k;
+    template <int al, class T>
+    __device__ __forceinline__ void put(T t) const {
+        constexpr int ey = EY[al], ez = EZ[al];
+        T* out = static_cast<T*>(a.halo_out);
+        bool ly = false, lz = false;
+        if constexpr (ey != 0) {
+            constexpr int leave = ey > 0 ? LBM_FACE_YP : LBM_FACE_YM, arrive = leave ^ 1;
+            constexpr int slot = cross_slot(arrive, al, -1);
+            ly = (ey > 0 ? j == a.ny - 1 : j == 0) && ((a.halo_faces >> leave) & 1);
+            if (ly) out[a.hoff[arrive] + (long long)slot * a.nx * a.nz + i + (long long)a.nx * k] = t;
+        }
+        if constexpr (ez != 0) {
+            constexpr int leave = ez > 0 ? LBM_FACE_ZP : LBM_FACE_ZM, arrive = leave ^ 1;
+            constexpr int slot = cross_slot(arrive, al, -1);
+            lz = (ez > 0 ? k == a.nz - 1 : k == 0) && ((a.halo_faces >> leave) & 1);
+            if (lz) out[a.hoff[arrive] + (long long)slot * a.nx * a.ny + i + (long long)a.nx * j] = t;
+        }
+        if constexpr (ey != 0 && ez != 0) {
+            if (ly && lz) out[a.hoff[edge_region(al)] + i] = t;
+        }
+    }
+};
+
+template <class S, int COLL, bool MACRO, bool LOOP>
 __global__ __launch_bounds__(256) void k_cavity_shell(const StepArgs a, const ShellRows r) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.nx) return;
@@ -77,7 +111,10 @@ __global__ __launch_bounds__(256) void k_cavity_shell(const StepArgs a, const Sh
         j = r.jf[q];
         k = r.k0 + (t - q * r.nzr);
     }
-    cavity_cell<S, COLL, MACRO, true>(a, i, j, k);
+    if constexpr (LOOP)
+        cavity_cell<S, COLL, MACRO, true>(a, i, j, k, LoopbackHalo<S>{a, i, j, k});
+    else
+        cavity_cell<S, COLL, MACRO, true>(a, i, j, k);
 }
 
 // Interior: j in [j0, j1), k in [k0, k0 + gridDim.z); no pull from these
@@ -308,9 +345,11 @@ struct Table {
         {{k_cavity_step<S, COLL_BGK, false, false>, k_cavity_step<S, COLL_BGK, false, true>},
          {k_cavity_step<S, COLL_BGK, true, false>, k_cavity_step<S, COLL_BGK, true, true>}}};
     static constexpr InitFn init[2] = {k_cavity_init<S, COLL_MRT>, k_cavity_init<S, COLL_BGK>};
-    static constexpr ShellFn shell[2][2] = {
-        {k_cavity_shell<S, COLL_MRT, false>, k_cavity_shell<S, COLL_MRT, true>},
-        {k_cavity_shell<S, COLL_BGK, false>, k_cavity_shell<S, COLL_BGK, true>}};
+    static constexpr ShellFn shell[2][2][2] = {
+        {{k_cavity_shell<S, COLL_MRT, false, false>, k_cavity_shell<S, COLL_MRT, false, true>},
+         {k_cavity_shell<S, COLL_MRT, true, false>, k_cavity_shell<S, COLL_MRT, true, true>}},
+        {{k_cavity_shell<S, COLL_BGK, false, false>, k_cavity_shell<S, COLL_BGK, false, true>},
+         {k_cavity_shell<S, COLL_BGK, true, false>, k_cavity_shell<S, COLL_BGK, true, true>}}};
     static constexpr InteriorFn interior[2][2] = {
         {k_cavity_interior<S, COLL_MRT, false>, k_cavity_interior<S, COLL_MRT, true>},
         {k_cavity_interior<S, COLL_BGK, false>, k_cavity_interior<S, COLL_BGK, true>}};
@@ -416,23 +455,34 @@ int launch_unpack(lbm_ctx* c, const void* recv, int loopback, unsigned mask = AL
     return lbm_check_launch();
 }
 
-// Regions whose cells all lie on halo faces: faces in halo_faces, and edges
-// whose two faces (the ones the diagonal population leaves through) both are.
-// With a face mask symmetric per axis these are also the regions read.
-unsigned shell_regions(const lbm_ctx* c) {
+// Regions a looped-back exchange must move.  A face region r is read when
+// face r is a halo face and is filled from the opposite face's outgoing
+// region r ^ 1; an edge region is read when both faces its population enters
+// through are halo faces and is filled from the same region.  With no halo
+// face the loopback rehearses the whole exchange (every region).
+void loopback_masks(const lbm_ctx* c, unsigned* pack, unsigned* unpack) {
     const int m = c->p.halo_faces;
-    unsigned mask = 0;
+    if (m == 0) {
+        *pack = *unpack = ALL_REGIONS;
+        return;
+    }
+    unsigned pm = 0, um = 0;
     for (int f = 0; f < NFACE; ++f)
-        if ((m >> f) & 1) mask |= 1u << f;
+        if ((m >> f) & 1) {
+            um |= 1u << f;
+            pm |= 1u << (f ^ 1);
+        }
     for (int al = 7; al < Q; ++al) {
         bool all = true;
-        for (int axis = 0; axis < 3; ++axis) {
-            const int e = ecomp(al, axis);
-            if (e != 0 && !((m >> (2 * axis + (e > 0 ? 1 : 0))) & 1)) all = false;
+        for (int axis = 0; axis < 3; ++axis)
+            if (ecomp(al, axis) != 0 && !((m >> entry_face(al, axis)) & 1)) all = false;
+        if (all) {
+            um |= 1u << edge_region(al);
+            pm |= 1u << edge_region(al);
         }
-        if (all) mask |= 1u << edge_region(al);
     }
-    return mask;
+    *pack = pm;
+    *unpack = um;
 }
 
 ShellRows shell_rows(const lbm_ctx* c) {
@@ -449,15 +499,15 @@ ShellRows shell_rows(const lbm_ctx* c) {
 }
 
 // Launch the two halves of one split step for args `a`.
-int launch_shell(lbm_ctx* c, const StepArgs& a, bool macro) {
+int launch_shell(lbm_ctx* c, const StepArgs& a, bool macro, bool loop) {
     const ShellRows r = shell_rows(c);
     if (r.rows == 0) return LBM_OK;
     const int coll = c->p.collision == LBM_BGK ? 1 : 0;
     ShellFn fn;
     switch (c->p.storage) {
-    case LBM_STORE_F32: fn = Table<StoreF32>::shell[coll][macro]; break;
-    case LBM_STORE_F16: fn = Table<StoreF16>::shell[coll][macro]; break;
-    default: fn = Table<StoreF64>::shell[coll][macro];
+    case LBM_STORE_F32: fn = Table<StoreF32>::shell[coll][macro][loop]; break;
+    case LBM_STORE_F16: fn = Table<StoreF16>::shell[coll][macro][loop]; break;
+    default: fn = Table<StoreF64>::shell[coll][macro][loop];
     }
     const int bx = c->block.x;
     hipLaunchKernelGGL(fn, dim3((c->p.nx + bx - 1) / bx, r.rows), dim3(bx), 0, c->stream, a, r);
@@ -481,42 +531,38 @@ int launch_interior(lbm_ctx* c, const StepArgs& a, bool macro) {
     return lbm_check_launch();
 }
 
-// nsteps split steps.  With the loopback the exchange of each step's shell
-// runs on the second stream while the interior is updated: the shell's
-// completion releases the pack, and the next step's shell waits for the
-// unpack.  The halo of the current state is exchanged first, on the main
-// stream; the last step's shell is not exchanged (the next call starts with
-// that exchange).
+// nsteps split steps: the shell, then the interior.  With the loopback the
+// shell also writes the next step's halo (LoopbackHalo) into the second halo
+// buffer, and the two buffers swap after each step; a halo that does not
+// belong to the current state (after lbm_initial, lbm_load_fields or
+// lbm_load_state) is exchanged once first.
 int step_split(lbm_ctx* c, int nsteps) {
     const bool loop = c->p.halo_loopback != 0;
-    const unsigned mask = shell_regions(c);
-    StepArgs a = make_args(c);
     int rc = LBM_OK;
-    if (loop && nsteps > 0) {
-        rc = launch_pack(c, c->buf[c->cur], c->hsend, mask);
-        if (!rc) rc = launch_unpack(c, c->hsend, 1, mask);
+    if (loop && nsteps > 0 && !c->halo_fresh) {
+        unsigned pmask, umask;
+        loopback_masks(c, &pmask, &umask);
+        rc = launch_pack(c, c->buf[c->cur], c->hsend, pmask);
+        if (!rc) rc = launch_unpack(c, c->hsend, 1, umask);
         if (rc) return rc;
+        c->halo_fresh = true;
     }
     for (int s = 0; s < nsteps; ++s) {
         const bool macro = s == nsteps - 1;
-        const bool exchange = loop && s < nsteps - 1;
+        StepArgs a = make_args(c);
         a.src = c->buf[c->cur];
         a.dst = c->buf[1 - c->cur];
         a.seed = (unsigned)(c->itc + s + 1);
-        rc = launch_shell(c, a, macro);
-        if (rc) return rc;
-        if (exchange) {
-            HIP_TRY(hipEventRecord(c->ev_shell, c->stream));
-            HIP_TRY(hipStreamWaitEvent(c->aux, c->ev_shell, 0));
-            rc = launch_pack(c, a.dst, c->hsend, mask, c->aux);
-            if (!rc) rc = launch_unpack(c, c->hsend, 1, mask, c->aux);
-            if (rc) return rc;
-            HIP_TRY(hipEventRecord(c->ev_halo, c->aux));
-        }
-        rc = launch_interior(c, a, macro);
+        a.halo_out = c->hin_next;
+        rc = launch_shell(c, a, macro, loop);
+        if (!rc) rc = launch_interior(c, a, macro);
         if (rc) return rc;
         c->cur = 1 - c->cur;
-        if (exchange) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_halo, 0));
+        if (loop) {
+            void* t = c->hin;
+            c->hin = c->hin_next;
+            c->hin_next = t;
+        }
     }
     return LBM_OK;
 }
@@ -541,9 +587,7 @@ void release(lbm_ctx* c) {
     if (c->hsend) (void)hipFree(c->hsend);
     thermal_release(c);
     particles_release(c);
-    if (c->ev_halo) (void)hipEventDestroy(c->ev_halo);
-    if (c->ev_shell) (void)hipEventDestroy(c->ev_shell);
-    if (c->aux) (void)hipStreamDestroy(c->aux);
+    if (c->hin_next) (void)hipFree(c->hin_next);
 }
 
 }  // namespace
@@ -724,14 +768,15 @@ int lbm_create(const lbm_params* p, lbm_ctx** out) {
             return rc == LBM_ERR_HIP ? LBM_ERR_NOMEM : rc;
         }
     }
-    if (p->halo_overlap &&
-        (hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
-         hipEventCreateWithFlags(&c->ev_shell, hipEventDisableTiming) != hipSuccess ||
-         hipEventCreateWithFlags(&c->ev_halo, hipEventDisableTiming) != hipSuccess)) {
-        (void)hipGetLastError();
-        release(c);
-        delete c;
-        return lbm_fail(LBM_ERR_HIP, "lbm_create: creating the exchange stream failed");
+    if (p->halo_overlap && p->halo_loopback) {
+        if (hipMalloc(&c->hin_next, c->elem * c->htotal) != hipSuccess ||
+            hipMemset(c->hin_next, 0, c->elem * c->htotal) != hipSuccess) {
+            (void)hipGetLastError();
+            release(c);
+            delete c;
+            return lbm_fail(LBM_ERR_NOMEM, "lbm_create: allocating the second halo buffer failed");
+        }
+        c->device_bytes += (long long)(c->elem * c->htotal);
     }
     // incoming halos start as zeros until the first exchange
     if (hipMemset(c->hin, 0, c->elem * c->htotal) != hipSuccess) {
@@ -785,6 +830,7 @@ static int load_device_fields(lbm_ctx* c, const double* d_rho, const double* d_u
 int lbm_initial(lbm_ctx* c) {
     if (!c) return lbm_fail(LBM_ERR_ARG, "lbm_initial: null context");
     c->in_step = false;  // a fresh start abandons a step in progress
+    c->halo_fresh = false;
     particles_release(c);  // a fresh start detaches any spheres
     // initial fields into the macro arrays, then equilibrium + first collision
     // from them.  The init kernel reads and rewrites the same macro values
@@ -813,6 +859,7 @@ int lbm_load_fields(lbm_ctx* c, const double* rho, const double* u, const double
                     const double* T) {
     if (!c || !rho || !u || !v || !w) return lbm_fail(LBM_ERR_ARG, "lbm_load_fields: null argument");
     c->in_step = false;
+    c->halo_fresh = false;
     particles_release(c);
     const size_t bytes = sizeof(double) * c->n;
     if (c->p.thermal) {
@@ -857,10 +904,12 @@ int lbm_step(lbm_ctx* c, int nsteps) {
     }
     StepArgs a = make_args(c);
     const dim3 grid = grid_of(c);
+    unsigned pmask, umask;
+    loopback_masks(c, &pmask, &umask);
     for (int s = 0; s < nsteps; ++s) {
         if (c->p.halo_loopback) {
-            int rc = launch_pack(c, c->buf[c->cur], c->hsend);
-            if (!rc) rc = launch_unpack(c, c->hsend, 1);
+            int rc = launch_pack(c, c->buf[c->cur], c->hsend, pmask);
+            if (!rc) rc = launch_unpack(c, c->hsend, 1, umask);
             if (rc) return rc;
         }
         a.src = c->buf[c->cur];
@@ -977,7 +1026,7 @@ int lbm_step_begin(lbm_ctx* c) {
     a.src = c->buf[c->cur];
     a.dst = c->buf[1 - c->cur];
     a.seed = (unsigned)(c->itc + 1);
-    int rc = launch_shell(c, a, true);
+    int rc = launch_shell(c, a, true, false);
     if (rc) return rc;
     c->in_step = true;
     return LBM_OK;

@@ -51,6 +51,7 @@ struct StoreF64 {
     static constexpr bool RANDOM = false;
     static constexpr int BYTES = 8;
     __device__ __forceinline__ static Real ld(const T* p, float) { return *p; }
+    __device__ __forceinline__ static T cvt(Real v, float, unsigned) { return v; }
     __device__ __forceinline__ static void st(T* p, Real v, float, unsigned) { __builtin_nontemporal_store(v, p); }
 };
 
@@ -61,6 +62,7 @@ struct StoreF32 {
     static constexpr bool RANDOM = false;
     static constexpr int BYTES = 4;
     __device__ __forceinline__ static Real ld(const T* p, float) { return *p; }
+    __device__ __forceinline__ static T cvt(Real v, float, unsigned) { return v; }
     __device__ __forceinline__ static void st(T* p, Real v, float, unsigned) { __builtin_nontemporal_store(v, p); }
 };
 
@@ -102,8 +104,12 @@ struct StoreF16 {
         // is the same (the scale is a power of two)
         return __builtin_fmaf(static_cast<float>(*p), inv_scale, 0.0f);
     }
+    // the stored value: v scaled, rounded stochastically to FP16
+    __device__ __forceinline__ static T cvt(Real v, float scale, unsigned rnd) {
+        return to_half_stochastic(v * scale, rnd);
+    }
     __device__ __forceinline__ static void st(T* p, Real v, float scale, unsigned rnd) {
-        __builtin_nontemporal_store(to_half_stochastic(v * scale, rnd), p);
+        __builtin_nontemporal_store(cvt(v, scale, rnd), p);
     }
 };
 

@@ -56,11 +56,12 @@ struct lbm_ctx {
     double* temp = nullptr;      // T (n)
     double* prev_t = nullptr;    // wp, Tp for the thermal check (2 * n)
     ParticleState* part = nullptr;  // moving spheres, when attached
-    // split steps (halo_overlap): the second stream the looped-back exchange
-    // runs on, the events that order it against the shell and the next step,
-    // and whether a step is between lbm_step_begin and lbm_step_end
-    hipStream_t aux = nullptr;
-    hipEvent_t ev_shell = nullptr, ev_halo = nullptr;
+    // split steps (halo_overlap): with the loopback, the halo the shell
+    // writes for the next step (swapped with hin after each step) and whether
+    // hin belongs to the current state; whether a step is between
+    // lbm_step_begin and lbm_step_end
+    void* hin_next = nullptr;
+    bool halo_fresh = false;
     bool in_step = false;
 };
 

@@ -154,6 +154,7 @@ int lbm_load_state(lbm_ctx* c, const void* blob, size_t bytes) {
     HIP_TRY(hipStreamSynchronize(c->stream));  // the caller may free the blob on return
     c->itc = h.itc;
     c->initialised = true;
+    c->halo_fresh = false;  // the next looped-back split step exchanges the loaded state's halo first
     return LBM_OK;
 }
 

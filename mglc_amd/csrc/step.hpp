@@ -33,6 +33,13 @@ struct StepArgs {
     long long hoff[NREGION];    // element offset of each region in the packed layout
     long long hcount[NREGION];  // elements in each region
     unsigned seed;              // step number: drives the stochastic rounding of FP16 stores
+    void* halo_out;             // split step with loopback: the halo of the next step (shell kernel)
+};
+
+// No extra use of a cell's stored populations.
+struct NoPost {
+    template <int al, class T>
+    __device__ __forceinline__ void put(T) const {}
 };
 
 enum { COLL_MRT = 0, COLL_BGK = 1 };
@@ -107,9 +114,13 @@ __device__ __forceinline__ typename S::Real pull_outside(const Args& a, const ty
 }
 
 // The update of cell (i, j, k): pull the 19 populations (bounce-back or halo
-// where the pull leaves the domain), macro, collision, store.
-template <class S, int COLL, bool MACRO, bool HALO, class Args>
-__device__ __forceinline__ void cavity_cell(const Args& a, int i, int j, int k) {
+// where the pull leaves the domain), macro, collision, store.  `post` sees
+// each stored value (post.put<a>(value)), e.g. to write it to a halo too,
+// after the last store: the collision and its stores stay one straight-line
+// block in every kernel, so the compiler contracts the arithmetic the same
+// way and every kernel built on this function produces the same bits.
+template <class S, int COLL, bool MACRO, bool HALO, class Args, class Post = NoPost>
+__device__ __forceinline__ void cavity_cell(const Args& a, int i, int j, int k, const Post& post = Post{}) {
     using T = typename S::T;
     using R = typename S::Real;
     const int c = i + a.px * (j + a.ny * k);
@@ -152,9 +163,15 @@ __device__ __forceinline__ void cavity_cell(const Args& a, int i, int j, int k) 
     }
     collide<S, COLL>(f, q, a);
     const unsigned h = cell_random<S>(c, a.seed);
+    T stored[Q];
     sfor<Q>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        S::st(dst + (long long)al * a.ps + c, f[al], a.scale, store_random<S, al>(h));
+        stored[al] = S::cvt(f[al], a.scale, store_random<S, al>(h));
+        __builtin_nontemporal_store(stored[al], dst + (long long)al * a.ps + c);
+    });
+    sfor<Q>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        post.template put<al>(stored[al]);
     });
 }
 
