@@ -8,7 +8,8 @@
 ! kernel behind lbm_step; steps between two checks go to the GPU as one call.
 !
 ! usage: mglc_cavity [params.nml]     (namelist &cavity, see commondata.f90)
-!        mglc_cavity --abi            print the size of lbm_params and exit
+!        mglc_cavity --abi            print the sizes of lbm_params, lbm_particle_params and
+!                                     lbm_info and exit
 program main
     use iso_c_binding
     use commondata
@@ -22,11 +23,15 @@ program main
     integer(8) :: t_start, t_end, t_rate
     real(8) :: wall
     character(len=16) :: arg1
+    type(lbm_particle_params) :: pp_abi
+    type(lbm_info) :: info_abi
 
     if (command_argument_count() >= 1) then
         call get_command_argument(1, arg1)
         if (trim(arg1) == '--abi') then
             write(*, '(i0)') c_sizeof(p)
+            write(*, '(i0)') c_sizeof(pp_abi)
+            write(*, '(i0)') c_sizeof(info_abi)
             stop
         endif
     endif

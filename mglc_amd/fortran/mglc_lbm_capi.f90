@@ -7,6 +7,11 @@
 !   lbm_download_macro the gather half of output.f90:1-133 (one GPU = one rank)
 !   lbm_save_state / lbm_load_state   backupData / loadInitField of the
 !                      buoyancy codes (seq/bouyancy3d.F90:1011-1030)
+!   lbm_halo_pack / lbm_halo_unpack   message_passing_sendrecv (ex_sendrecv.f90)
+!   lbm_step_begin / lbm_step_end     collision_with_message_exchange
+!                      (mpi_3d_nonblocked/lid3_mpi_nonblock.f90:1108-1231)
+!   lbm_check_thermal, lbm_nusselt, lbm_reynolds   buoyancy check.F90, RaNu.F90
+!   lbm_particles_*    the moving-sphere chain of Micro_particles case4
 module mglc_lbm_capi
     use iso_c_binding
     implicit none
@@ -37,6 +42,19 @@ module mglc_lbm_capi
         real(c_double) :: t_hot, t_cold, t_ref
         real(c_double) :: reserved_d(2)
     end type lbm_params
+
+    ! mirrors struct lbm_particle_params
+    type, bind(C) :: lbm_particle_params
+        integer(c_int) :: np, reserved_i
+        real(c_double) :: rho_solid, gravity
+        real(c_double) :: threshold_wall, stiff_wall
+        real(c_double) :: threshold_particle, stiff_particle
+        real(c_double) :: reserved_d(4)
+    end type lbm_particle_params
+
+    integer(c_int), parameter :: LBM_FACE_XM = 0, LBM_FACE_XP = 1, LBM_FACE_YM = 2, LBM_FACE_YP = 3
+    integer(c_int), parameter :: LBM_FACE_ZM = 4, LBM_FACE_ZP = 5, LBM_NUM_REGIONS = 18
+    integer(c_int), parameter :: LBM_ERR_UNSUPPORTED = 7, LBM_ERR_PARTICLE = 8
 
     ! mirrors struct lbm_info
     type, bind(C) :: lbm_info
@@ -140,6 +158,75 @@ module mglc_lbm_capi
             type(c_ptr), value :: ctx
             type(lbm_info), intent(out) :: info
         end function lbm_get_info
+
+        integer(c_int) function lbm_set_stream(ctx, hip_stream) bind(C, name='lbm_set_stream')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, hip_stream
+        end function lbm_set_stream
+
+        integer(c_int) function lbm_check_sums(ctx, e1, e2) bind(C, name='lbm_check_sums')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: e1, e2
+        end function lbm_check_sums
+
+        ! the subdomain exchange (ex_sendrecv.f90:12-123): device buffers
+        integer(c_int) function lbm_halo_layout(ctx, region, offset, count) bind(C, name='lbm_halo_layout')
+            import :: c_int, c_ptr, c_long_long
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: region
+            integer(c_long_long), intent(out) :: offset, count
+        end function lbm_halo_layout
+
+        integer(c_int) function lbm_halo_buffer_bytes(ctx, bytes) bind(C, name='lbm_halo_buffer_bytes')
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), value :: ctx
+            integer(c_size_t), intent(out) :: bytes
+        end function lbm_halo_buffer_bytes
+
+        integer(c_int) function lbm_halo_pack(ctx, dev_send) bind(C, name='lbm_halo_pack')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, dev_send
+        end function lbm_halo_pack
+
+        integer(c_int) function lbm_halo_unpack(ctx, dev_recv) bind(C, name='lbm_halo_unpack')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, dev_recv
+        end function lbm_halo_unpack
+
+        ! collision_with_message_exchange (lid3_mpi_nonblock.f90:1108-1231):
+        ! boundary shell, then the interior
+        integer(c_int) function lbm_step_begin(ctx) bind(C, name='lbm_step_begin')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function lbm_step_begin
+
+        integer(c_int) function lbm_step_end(ctx) bind(C, name='lbm_step_end')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function lbm_step_end
+
+        ! moving spheres (Micro_particles/fortran/case4/mpi: initial.F90:78-135,
+        ! the particle state of commondata.F90:14-35, obst)
+        integer(c_int) function lbm_particles_init(ctx, pp, centers, radii, U, omega) &
+                bind(C, name='lbm_particles_init')
+            import :: c_int, c_ptr, c_double, lbm_particle_params
+            type(c_ptr), value :: ctx
+            type(lbm_particle_params), intent(in) :: pp
+            real(c_double), intent(in) :: centers(*), radii(*), U(*), omega(*)
+        end function lbm_particles_init
+
+        integer(c_int) function lbm_particles_get(ctx, centers, U, omega, F, T) bind(C, name='lbm_particles_get')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: centers(*), U(*), omega(*), F(*), T(*)
+        end function lbm_particles_get
+
+        integer(c_int) function lbm_particles_map(ctx, owner) bind(C, name='lbm_particles_map')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), intent(out) :: owner(*)
+        end function lbm_particles_map
 
         type(c_ptr) function lbm_last_error_c() bind(C, name='lbm_last_error')
             import :: c_ptr

@@ -65,11 +65,25 @@ def _driver():
 
 
 def test_fortran_driver_abi_matches():
-    """The Fortran bind(C) mirror of lbm_params has the C struct's size."""
+    """The Fortran bind(C) mirrors of lbm_params, lbm_particle_params and
+    lbm_info have the C structs' sizes."""
     import subprocess
     from mglc_amd import lbm
-    out = subprocess.run([_driver(), "--abi"], capture_output=True, text=True, check=True).stdout
-    assert int(out.split()[0]) == ctypes.sizeof(lbm.LbmParams)
+    out = subprocess.run([_driver(), "--abi"], capture_output=True, text=True, check=True).stdout.split()
+    assert int(out[0]) == ctypes.sizeof(lbm.LbmParams)
+    assert int(out[1]) == ctypes.sizeof(lbm.LbmParticleParams)
+    assert int(out[2]) == ctypes.sizeof(lbm.LbmInfo)
+
+
+def test_fortran_module_binds_every_single_context_entry_point():
+    """mglc_lbm_capi.f90 has a bind(C) interface for every entry point of a
+    single context (the batch API and the bandwidth probe are host-language
+    conveniences; lbm_status_string is not needed by the drivers)."""
+    text = open(os.path.join(ROOT, "mglc_amd", "fortran", "mglc_lbm_capi.f90")).read()
+    bound = set(re.findall(r"bind\(C, name='(lbm_\w+)'\)", text))
+    want = {f for f in declared_functions()
+            if not f.startswith("lbm_batch_") and f not in ("lbm_bandwidth_probe", "lbm_status_string")}
+    assert want <= bound, sorted(want - bound)
 
 
 def test_fortran_driver_binds_only_declared_symbols():
