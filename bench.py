@@ -212,6 +212,25 @@ def cpu_baseline_cavity(args):
                       f"initial state, oracle/lbm_oracle.c with {threads} OpenMP threads, loop time only"}
 
 
+def cpu_reference(kind, n, steps, cells):
+    """The reference's own subroutines (oracle/_ref, built from the reference's
+    Fortran sources, one process) timed on `steps` steps of the same n^3 grid;
+    None where that build is absent."""
+    from oracle import oracle as O
+    exe = O.REF_CAVITY if kind == "cavity" else O.REF_THERMAL
+    if not os.path.exists(exe):
+        return None
+    sec = O.reference_loop_seconds(kind, n, n, n, steps)
+    what = ("MPI/Lid_driven_cavity/fortran/3d/mpi_3d_blocked collision, streaming, bounceback, macro"
+            if kind == "cavity" else
+            "MPI/Buoyancy_driven_cavity/fortran/3d/mpi_blocked collision, streaming, bounceback, collisionT, "
+            "streamingT, bouncebackT, macro, macroT")
+    return {"value": cells * steps / sec / 1e6, "unit": "MLUPS", "cores": 1, "kind": "reference",
+            "sample": f"{n}^3, {steps} steps of the reference's own {what} (amdflang -O3, one process, "
+                      f"loop time only; its compile-time tau and Ra differ from this workload's, which "
+                      f"changes no operation count)"}
+
+
 def bench_cavity(args, stream):
     primary = run_storage(args, "f64", stream)
     variants = {}
@@ -252,6 +271,9 @@ def bench_cavity(args, stream):
             v["attainable"] = attainable(st, args.n, v["kernel_hbm_frac"] * HBM_PEAK_GBS)
     if args.cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_cavity(args)
+        ref = cpu_reference("cavity", args.n, 2, args.n ** 3)
+        if ref is not None:
+            line["cpu_reference"] = ref
     return line
 
 
@@ -445,6 +467,9 @@ def bench_thermal(args, stream):
         line["cpu_baseline"] = {"value": rate, "unit": "MLUPS", "cores": threads, "kind": "port",
                                 "sample": f"the same {n}^3 case in FP64, {steps} steps, oracle/thermal_oracle.c "
                                           f"with {threads} OpenMP threads, set-up time subtracted"}
+        ref = cpu_reference("thermal", n, 6, cells)
+        if ref is not None:
+            line["cpu_reference"] = ref
     return line
 
 

@@ -121,6 +121,22 @@ def run_reference_cavity(nx, ny, nz, nsteps, outfile):
     return dict(tauf=head[0], U0=head[1], rho0=head[2]), rho, u, v, w
 
 
+def reference_loop_seconds(kind, nx, ny, nz, nsteps):
+    """Wall time of the step loop of the reference's own subroutines
+    (oracle/_ref/ref_cavity or ref_thermal, one process, initial and output
+    excluded) over nsteps steps: the CPU baseline of the reference's code."""
+    import tempfile
+    exe = REF_CAVITY if kind == "cavity" else REF_THERMAL
+    with tempfile.TemporaryDirectory() as d:
+        out = subprocess.run([exe, str(nx), str(ny), str(nz), str(nsteps), os.path.join(d, "fields.bin")],
+                             check=True, capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        parts = line.split()
+        if len(parts) == 2 and parts[0] == "ref_loop_seconds":
+            return float(parts[1])
+    raise RuntimeError(f"{exe} printed no ref_loop_seconds line")
+
+
 # ---- buoyancy-driven cavity (D3Q19 + Guo forcing, D3Q7 temperature) ----------
 REF_THERMAL = os.path.join(HERE, "_ref", "ref_thermal")
 

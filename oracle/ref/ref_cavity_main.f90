@@ -9,11 +9,14 @@
 ! product calls it.  tauf, U0 and Reynolds are the reference's compile-time
 ! parameters (commondata.f90:4-9); only nx, ny, nz and the step count vary.
 !
+! Prints 'ref_loop_seconds <s>', the wall time of the step loop.
+!
 ! usage: ref_cavity nx ny nz nsteps outfile
 program ref_cavity_main
     use commondata
     implicit none
     integer :: nsteps, s
+    integer(8) :: t0, t1, rate
     character(len=256) :: arg, outfile
 
     call get_command_argument(1, arg); read(arg, *) nx
@@ -28,6 +31,7 @@ program ref_cavity_main
     coords = 0
 
     call initial()
+    call system_clock(t0, rate)
     do s = 1, nsteps
         itc = itc + 1
         call collision()
@@ -35,6 +39,9 @@ program ref_cavity_main
         call bounceback()
         call macro()
     enddo
+    call system_clock(t1)
+    ! the loop's wall time, for the CPU baseline (initial and output excluded)
+    write(*, '(a, es24.16)') 'ref_loop_seconds ', dble(t1 - t0) / dble(rate)
 
     open(10, file=trim(outfile), form='unformatted', access='stream', status='replace')
     write(10) tauf, U0, rho0

@@ -10,11 +10,14 @@
 ! Mach, Ekman and the wall temperatures are the reference's compile-time
 ! parameters (module.F90:11-33); only nx, ny, nz and the step count vary.
 !
+! Prints 'ref_loop_seconds <s>', the wall time of the step loop.
+!
 ! usage: ref_thermal nx ny nz nsteps outfile
 program ref_thermal_main
     use commondata
     implicit none
     integer :: nsteps, s
+    integer(8) :: t0, t1, rate
     character(len=256) :: arg, outfile
 
     call get_command_argument(1, arg); read(arg, *) nx
@@ -28,6 +31,7 @@ program ref_thermal_main
     coords = 0
 
     call initial()
+    call system_clock(t0, rate)
     do s = 1, nsteps
         itc = itc + 1
         call collision()
@@ -39,6 +43,9 @@ program ref_thermal_main
         call macro()
         call macroT()
     enddo
+    call system_clock(t1)
+    ! the loop's wall time, for the CPU baseline (initial and output excluded)
+    write(*, '(a, es24.16)') 'ref_loop_seconds ', dble(t1 - t0) / dble(rate)
 
     open(10, file=trim(outfile), form='unformatted', access='stream', status='replace')
     write(10) tauf, paraA, gBeta, omegaRatating, Thot, Tcold, Tref, diffusivity
