@@ -116,14 +116,19 @@ def roofline(bytes_per_launch, ms_per_launch, traffic=None):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
 
 
-def attainable(storage, n, achieved):
+def attainable(storage, n, achieved, thermal=False):
     """What the memory system delivers for this traffic with no arithmetic,
     measured now on this device: a one-stream copy of the same bytes and the
-    D3Q19 pull pattern of an n^3 lattice (19 upstream loads, 19 stores per
-    cell) in the library's layout; the kernel's achieved rate as a fraction
-    of the latter."""
+    kernel's own pull pattern on an n^3 lattice in the library's layout (the
+    D3Q19 pull: 19 upstream loads and 19 stores per cell; with `thermal` the
+    buoyancy step's D3Q19 + D3Q7 pulls and force plane, 27 and 27); the
+    kernel's achieved rate as a fraction of the latter."""
     from mglc_amd.lbm import bandwidth_probe
     copy = bandwidth_probe(0, storage, n)
+    if thermal:
+        pull = bandwidth_probe(2, storage, n)
+        return {"copy_one_stream_GBs": round(copy, 1), "pull_d3q19_d3q7_force_no_arithmetic_GBs": round(pull, 1),
+                "kernel_frac_of_pattern": round(achieved / pull, 4), "probe_n": n, "storage": storage}
     pull = bandwidth_probe(1, storage, n)
     return {"copy_one_stream_GBs": round(copy, 1), "pull19_no_arithmetic_GBs": round(pull, 1),
             "kernel_frac_of_pull19": round(achieved / pull, 4), "probe_n": n, "storage": storage}
@@ -448,6 +453,8 @@ def bench_thermal(args, stream):
     line = line_head(args, mlups, ms / args.steps, "f64",
                      "synthetic (rho=1, u=0, T=0 plus seeded noise +-1e-4, held wall layers)", config)
     line["roofline"] = roofline(bpl * cells, ms / args.steps, traffic_from_profiles(f"thermal{n} f64"))
+    if args.attainable:
+        line["roofline"]["attainable"] = attainable("f64", n, line["roofline"]["achieved"], thermal=True)
     line["nusselt"] = {"at_step": max(args.warmup + args.steps, args.accuracy_steps), "nu_volume": nu_vol,
                        "nu_wall": nu_wall}
     if args.cpu_baseline:

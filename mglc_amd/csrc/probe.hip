@@ -10,6 +10,9 @@
 //              upstream neighbours (population planes laid out as in
 //              lbm_create: x padded to 16, whole 4 KiB pages plus a 1 KiB
 //              stagger per plane) and writes them to its own cell
+//   pattern 2: the buoyancy step's traffic (FP64 only): the D3Q19 pull, the
+//              D3Q7 pull of the temperature populations and one force plane
+//              read and rewritten in place (27 planes in, 27 out)
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -28,35 +31,40 @@ __global__ __launch_bounds__(256) void k_probe_copy(const uint64_t* __restrict__
 }
 
 // src and dst point `pad` elements into their allocations; pad exceeds the
-// largest pull offset, so every read stays inside the allocation.
-template <class T>
+// largest pull offset, so every read stays inside the allocation.  Planes
+// [0, 19) are pulled as D3Q19, the next Q7P planes as D3Q7 (the first seven
+// velocities), the last PLAIN planes are read and written at the cell itself.
+template <class T, int Q7P, int PLAIN>
 __global__ __launch_bounds__(256) void k_probe_pull(const T* __restrict__ src, T* __restrict__ dst, int nx, int ny,
                                                     int px, long long pxy, long long ps) {
+    constexpr int NP = Q + Q7P + PLAIN;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= nx || j >= ny) return;
     const long long c = i + (long long)px * j + pxy * k;
-    T v[Q];
-    sfor<Q>([&](auto A) {
-        constexpr int al = decltype(A)::value;
+    T v[NP];
+    sfor<NP>([&](auto A) {
+        constexpr int p = decltype(A)::value;
+        constexpr int al = p < Q ? p : (p < Q + Q7P ? p - Q : 0);
         const long long off = (long long)EX[al] + (long long)EY[al] * px + (long long)EZ[al] * pxy;
-        v[al] = src[al * ps + c - off];
+        v[p] = src[p * ps + c - off];
     });
-    sfor<Q>([&](auto A) {
-        constexpr int al = decltype(A)::value;
-        dst[al * ps + c] = v[al];
+    sfor<NP>([&](auto A) {
+        constexpr int p = decltype(A)::value;
+        dst[p * ps + c] = v[p];
     });
 }
 
 template <class T>
 int probe(int pattern, int n, int reps, double* gbps) {
+    const int planes = pattern == 2 ? Q + 7 + 1 : Q;
     const int px = (n + 15) / 16 * 16;
     const long long pxy = (long long)px * n, cells = pxy * n;
     const long long page = 4096 / (long long)sizeof(T);
     const long long ps = (cells + page - 1) / page * page + 1024 / (long long)sizeof(T);
     const long long pad = (pxy + px + 1 + page - 1) / page * page;
-    const long long elems = Q * ps + 2 * pad;
+    const long long elems = planes * ps + 2 * pad;
     const long long n3 = (long long)n * n * n;
     T* a = nullptr;
     T* b = nullptr;
@@ -87,8 +95,12 @@ int probe(int pattern, int n, int reps, double* gbps) {
             if (pattern == 0)
                 hipLaunchKernelGGL(k_probe_copy, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s,
                                    (const uint64_t*)src, (uint64_t*)dst, words);
+            else if (pattern == 1)
+                hipLaunchKernelGGL((k_probe_pull<T, 0, 0>), grid, block, 0, s, src + pad, dst + pad, n, n, px, pxy,
+                                   ps);
             else
-                hipLaunchKernelGGL(k_probe_pull<T>, grid, block, 0, s, src + pad, dst + pad, n, n, px, pxy, ps);
+                hipLaunchKernelGGL((k_probe_pull<T, 7, 1>), grid, block, 0, s, src + pad, dst + pad, n, n, px, pxy,
+                                   ps);
         };
         for (int r = 0; r < 3; ++r) launch(r);
         (void)hipEventRecord(e0, s);
@@ -99,7 +111,7 @@ int probe(int pattern, int n, int reps, double* gbps) {
             hipEventElapsedTime(&ms, e0, e1) != hipSuccess || !(ms > 0.0f))
             rc = lbm_fail(LBM_ERR_HIP, "lbm_bandwidth_probe: timing failed");
         else
-            *gbps = (pattern == 0 ? 16.0 * (double)words : 2.0 * Q * (double)sizeof(T) * (double)n3) /
+            *gbps = (pattern == 0 ? 16.0 * (double)words : 2.0 * planes * (double)sizeof(T) * (double)n3) /
                     (ms * 1e-3 / reps) / 1e9;
     }
     if (e1) (void)hipEventDestroy(e1);
@@ -114,7 +126,9 @@ int probe(int pattern, int n, int reps, double* gbps) {
 
 extern "C" int lbm_bandwidth_probe(int pattern, int storage, int n, int reps, double* gbps) {
     if (!gbps) return lbm_fail(LBM_ERR_ARG, "lbm_bandwidth_probe: null result pointer");
-    if (pattern != 0 && pattern != 1) return lbm_fail(LBM_ERR_ARG, "lbm_bandwidth_probe: pattern %d", pattern);
+    if (pattern < 0 || pattern > 2) return lbm_fail(LBM_ERR_ARG, "lbm_bandwidth_probe: pattern %d", pattern);
+    if (pattern == 2 && storage != LBM_STORE_F64)
+        return lbm_fail(LBM_ERR_ARG, "lbm_bandwidth_probe: the buoyancy pattern is FP64");
     if (n < 16 || n > 512) return lbm_fail(LBM_ERR_ARG, "lbm_bandwidth_probe: n = %d outside [16, 512]", n);
     if (reps < 1 || reps > 1000) return lbm_fail(LBM_ERR_ARG, "lbm_bandwidth_probe: reps = %d", reps);
     switch (storage) {

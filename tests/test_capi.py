@@ -97,7 +97,8 @@ def test_fortran_driver_binds_only_declared_symbols():
 def test_bandwidth_probe_rejects_bad_arguments(hip_lib):
     """lbm_bandwidth_probe validates before any device call."""
     g = ctypes.c_double()
-    assert hip_lib.lbm_bandwidth_probe(2, 0, 64, 1, ctypes.byref(g)) == 1
+    assert hip_lib.lbm_bandwidth_probe(3, 0, 64, 1, ctypes.byref(g)) == 1
+    assert hip_lib.lbm_bandwidth_probe(2, 1, 64, 1, ctypes.byref(g)) == 1
     assert hip_lib.lbm_bandwidth_probe(0, 3, 64, 1, ctypes.byref(g)) == 1
     assert hip_lib.lbm_bandwidth_probe(1, 0, 8, 1, ctypes.byref(g)) == 1
     assert hip_lib.lbm_bandwidth_probe(1, 0, 64, 0, ctypes.byref(g)) == 1

@@ -15,3 +15,9 @@ def test_probe_rates_are_plausible(storage):
     pull = bandwidth_probe(1, storage, 96, reps=5)
     assert 50.0 < copy < 20000.0
     assert 50.0 < pull < 20000.0
+
+
+def test_probe_buoyancy_pattern():
+    print("[gpu] bandwidth probe buoyancy pattern 96^3", flush=True)
+    g = bandwidth_probe(2, "f64", 96, reps=5)
+    assert 50.0 < g < 20000.0
