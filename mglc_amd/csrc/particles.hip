@@ -28,7 +28,9 @@
 //   k_particle_mark   new solid map, rho_s and u = 0 in solids
 //                     (particle_update.F90:69-91)
 //   k_fluid_sum/mean  rhoAvg over the new fluid nodes
-//   k_particle_refill one block per sphere: uncovered nodes refilled by 3-point
+//   k_refill_scan     one block per sphere: its uncovered nodes (old solid,
+//                     new fluid) appended to a list
+//   k_refill_proc     one lane per listed node: refilled by 3-point
 //                     extrapolation along the most outward lattice direction
 //                     and moment replacement with the body velocity
 //                     (particle_update.F90:110-186), then collided
@@ -70,6 +72,9 @@ struct ParticleState {
     int* wave_sphere = nullptr;     // max_waves: the sphere of each wave
     double* partial = nullptr;      // 6 per wave: force and torque sums
     long long max_waves = 0;
+    int* refill = nullptr;        // refill_cap pairs (node, sphere) of uncovered nodes
+    int* refill_count = nullptr;  // entries this step
+    int refill_cap = 0;
     double* scal = nullptr;  // [0] rhoAvg of the streaming, [1] rhoAvg of the refill
     double* red = nullptr;   // reduction partials
     int* err = nullptr;      // device stop record {code, step}; code 0 = running
@@ -79,6 +84,7 @@ struct ParticleState {
 namespace {
 
 constexpr double PI = 3.14159265358979323846;
+constexpr int REFILL_BLOCKS = 128;  // k_refill_proc: lanes walk the list with a grid stride
 
 struct PArgs {
     const void* src;
@@ -112,6 +118,10 @@ struct PArgs {
     double gravity, threshold_wall, stiff_wall, threshold_particle, stiff_particle;
     double *F, *T, *Ftot;
     double *xc_out, *U_out, *Om_out;
+    // uncovered nodes to refill
+    int* refill;
+    int* refill_count;
+    int refill_cap;
 };
 
 using lbm::calq;
@@ -504,6 +514,7 @@ __device__ double streamed_pop(const PArgs& a, const typename S::T* src, int i, 
 __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
     const int lane = threadIdx.x & 63;
     const int p = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.refill_count = 0;  // the refill list of this step starts empty
     // p and the stop flag are uniform over the wave (the flag read by lane 0),
     // so the whole wave takes part in the shuffles below
     if (p >= a.np || __builtin_amdgcn_readfirstlane(stopped(a) ? 1 : 0)) return;
@@ -587,10 +598,12 @@ __global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner
     const int id = blockIdx.x;
     for_sphere_nodes(a, a.xc_new, id, [&](int, int, int, long long m) {
         atomicMax(owner_out + m, id);  // a node claimed twice goes to the higher index
-        a.rho[m] = a.rho_solid;
-        a.u[m] = 0.0;
-        a.v[m] = 0.0;
-        a.w[m] = 0.0;
+        if (a.owner[m] < 0) {  // newly covered; nodes solid before already hold rho_s, u = 0
+            a.rho[m] = a.rho_solid;
+            a.u[m] = 0.0;
+            a.v[m] = 0.0;
+            a.w[m] = 0.0;
+        }
     });
 }
 
@@ -636,70 +649,96 @@ __global__ __launch_bounds__(256) void k_particle_init_fields(const PArgs a) {
     });
 }
 
+// Refill of one uncovered node (i, j, k) = m of sphere id: inside the
+// sphere's old position, fluid in the new map.
 template <class S, int COLL>
-__global__ __launch_bounds__(256) void k_particle_refill(const PArgs a) {
+__device__ __forceinline__ void refill_node(const PArgs& a, int i, int j, int k, long long m, int id) {
     using T = typename S::T;
     using R = typename S::Real;
-    if (stopped(a)) return;
-    const int id = blockIdx.x;
     const T* src = static_cast<const T*>(a.src);
     const double rho_avg = a.rho_avg[1];
-    // uncovered nodes: inside the sphere's old position, fluid in the new map
-    for_sphere_nodes(a, a.xc, id, [&](int i, int j, int k, long long m) {
+    // extrapolation direction: most outward lattice direction w.r.t. the new centre
+    const double rx = i + 1.0 - a.xc_new[3 * id], ry = j + 1.0 - a.xc_new[3 * id + 1],
+                 rz = k + 1.0 - a.xc_new[3 * id + 2];
+    const double rn = sqrt(dist2(rx, ry, rz));
+    double best = 0.0;
+    int ec = 0;
+    for (int al = 1; al < Q; ++al) {
+        const double tn = __dadd_rn(__dadd_rn(rx * EX[al], ry * EY[al]), rz * EZ[al]) / rn;
+        if (tn > best) {
+            best = tn;
+            ec = al;
+        }
+    }
+    if (ec == 0) {
+        stop(a, 4);
+        return;
+    }
+    double fr[Q];
+    for (int al = 0; al < Q; ++al) fr[al] = 0.0;
+    for (int s = 1; s <= 3; ++s) {
+        const int pi = i + s * EX[ec], pj = j + s * EY[ec], pk = k + s * EZ[ec];
+        bool ok = in_box(a, pi, pj, pk);
+        if (ok) {
+            const long long pm = pi + (long long)a.nx * (pj + (long long)a.ny * pk);
+            ok = a.owner[pm] < 0 && a.owner_new[pm] < 0;
+        }
+        const double coef = s == 1 ? 3.0 : (s == 2 ? -3.0 : 1.0);
+        for (int al = 0; al < Q; ++al) {
+            const double v = ok ? streamed_pop<S>(a, src, pi, pj, pk, al) : weight(al) * a.rho0;
+            fr[al] = s == 1 ? coef * v : fr[al] + coef * v;
+        }
+    }
+    // momentum moments replaced by rhoAvg times the body velocity
+    double mm[Q];
+    mrt_forward(fr, mm);
+    double ux, uy, uz;
+    body_velocity(a.xc_new, a.U_new, a.Om_new, id, i + 1.0, j + 1.0, k + 1.0, ux, uy, uz);
+    mm[3] = rho_avg * ux;
+    mm[5] = rho_avg * uy;
+    mm[7] = rho_avg * uz;
+    mrt_inverse(mm, fr);
+    R f[Q];
+    for (int al = 0; al < Q; ++al) f[al] = S::SHIFT ? R(fr[al] - weight(al)) : R(fr[al]);
+    const Moments<R, S::SHIFT> mo = moments<R, S::SHIFT>(f);
+    a.rho[m] = S::SHIFT ? 1.0 + double(mo.drho) : double(mo.rho);
+    a.u[m] = double(mo.u);
+    a.v[m] = double(mo.v);
+    a.w[m] = double(mo.w);
+    collide<S, COLL>(f, mo, a);
+    const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
+    T* __restrict__ dst = static_cast<T*>(a.dst);
+    for (int al = 0; al < Q; ++al) S::st(dst + (long long)al * a.ps + c, f[al], a.scale, 0u);
+}
+
+// The uncovered nodes of each sphere, appended to the refill list.  The
+// list's order does not matter: a node's refill reads only the previous step's
+// populations, the maps and the sphere states, and writes only its own cell.
+__global__ __launch_bounds__(256) void k_refill_scan(const PArgs a) {
+    if (stopped(a)) return;
+    const int id = blockIdx.x;
+    for_sphere_nodes(a, a.xc, id, [&](int, int, int, long long m) {
         if (!(a.owner[m] == id && a.owner_new[m] < 0)) return;
-        // extrapolation direction: most outward lattice direction w.r.t. the new centre
-        const double rx = i + 1.0 - a.xc_new[3 * id], ry = j + 1.0 - a.xc_new[3 * id + 1],
-                     rz = k + 1.0 - a.xc_new[3 * id + 2];
-        const double rn = sqrt(dist2(rx, ry, rz));
-        double best = 0.0;
-        int ec = 0;
-        for (int al = 1; al < Q; ++al) {
-            const double tn = __dadd_rn(__dadd_rn(rx * EX[al], ry * EY[al]), rz * EZ[al]) / rn;
-            if (tn > best) {
-                best = tn;
-                ec = al;
-            }
+        const int slot = atomicAdd(a.refill_count, 1);
+        if (slot < a.refill_cap) {
+            a.refill[2 * slot] = (int)m;
+            a.refill[2 * slot + 1] = id;
+        } else {
+            stop(a, 6);
         }
-        if (ec == 0) {
-            stop(a, 4);
-            return;
-        }
-        double fr[Q];
-        for (int al = 0; al < Q; ++al) fr[al] = 0.0;
-        for (int s = 1; s <= 3; ++s) {
-            const int pi = i + s * EX[ec], pj = j + s * EY[ec], pk = k + s * EZ[ec];
-            bool ok = in_box(a, pi, pj, pk);
-            if (ok) {
-                const long long pm = pi + (long long)a.nx * (pj + (long long)a.ny * pk);
-                ok = a.owner[pm] < 0 && a.owner_new[pm] < 0;
-            }
-            const double coef = s == 1 ? 3.0 : (s == 2 ? -3.0 : 1.0);
-            for (int al = 0; al < Q; ++al) {
-                const double v = ok ? streamed_pop<S>(a, src, pi, pj, pk, al) : weight(al) * a.rho0;
-                fr[al] = s == 1 ? coef * v : fr[al] + coef * v;
-            }
-        }
-        // momentum moments replaced by rhoAvg times the body velocity
-        double mm[Q];
-        mrt_forward(fr, mm);
-        double ux, uy, uz;
-        body_velocity(a.xc_new, a.U_new, a.Om_new, id, i + 1.0, j + 1.0, k + 1.0, ux, uy, uz);
-        mm[3] = rho_avg * ux;
-        mm[5] = rho_avg * uy;
-        mm[7] = rho_avg * uz;
-        mrt_inverse(mm, fr);
-        R f[Q];
-        for (int al = 0; al < Q; ++al) f[al] = S::SHIFT ? R(fr[al] - weight(al)) : R(fr[al]);
-        const Moments<R, S::SHIFT> mo = moments<R, S::SHIFT>(f);
-        a.rho[m] = S::SHIFT ? 1.0 + double(mo.drho) : double(mo.rho);
-        a.u[m] = double(mo.u);
-        a.v[m] = double(mo.v);
-        a.w[m] = double(mo.w);
-        collide<S, COLL>(f, mo, a);
-        const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
-        T* __restrict__ dst = static_cast<T*>(a.dst);
-        for (int al = 0; al < Q; ++al) S::st(dst + (long long)al * a.ps + c, f[al], a.scale, 0u);
     });
+}
+
+template <class S, int COLL>
+__global__ __launch_bounds__(256) void k_refill_proc(const PArgs a) {
+    if (stopped(a)) return;
+    const int count = min(*a.refill_count, a.refill_cap);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
+        const long long m = a.refill[2 * t];
+        const int id = a.refill[2 * t + 1];
+        const int i = (int)(m % a.nx), j = (int)((m / a.nx) % a.ny), k = (int)(m / ((long long)a.nx * a.ny));
+        refill_node<S, COLL>(a, i, j, k, m, id);
+    }
 }
 
 // Sum of rho and count over fluid nodes, then rhoAvg into out.
@@ -809,6 +848,9 @@ PArgs particle_args(const lbm_ctx* c) {
     a.wave_off = s->wave_off;
     a.wave_sphere = s->wave_sphere;
     a.partial = s->partial;
+    a.refill = s->refill;
+    a.refill_count = s->refill_count;
+    a.refill_cap = s->refill_cap;
     return a;
 }
 
@@ -846,7 +888,8 @@ void particle_launches(lbm_ctx* c, int step, bool macro) {
     hipLaunchKernelGGL(k_fill_int, dim3(2048), dim3(256), 0, c->stream, s->owner[1 - s->cur], c->n, -1);
     hipLaunchKernelGGL(k_particle_mark, dim3(np), dim3(256), 0, c->stream, a, s->owner[1 - s->cur]);
     launch_fluid_mean(c, s->owner[1 - s->cur], s->scal + 1);
-    hipLaunchKernelGGL((k_particle_refill<S, COLL>), dim3(np), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_refill_scan, dim3(np), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL((k_refill_proc<S, COLL>), dim3(REFILL_BLOCKS), dim3(256), 0, c->stream, a);
     s->cur = 1 - s->cur;
     c->cur = 1 - c->cur;
 }
@@ -860,7 +903,7 @@ void free_state(ParticleState* s) {
     }
     for (void* p : {(void*)s->r, (void*)s->F, (void*)s->T, (void*)s->Ftot, (void*)s->items, (void*)s->slab_off,
                     (void*)s->slab_cap, (void*)s->link_count, (void*)s->wave_off, (void*)s->wave_sphere, (void*)s->partial, (void*)s->scal,
-                    (void*)s->red, (void*)s->err})
+                    (void*)s->red, (void*)s->err, (void*)s->refill, (void*)s->refill_count})
         if (p) (void)hipFree(p);
 }
 
@@ -900,9 +943,9 @@ int particles_step(lbm_ctx* c, int nsteps) {
         "a particle penetrates a wall (particle_force.F90:136-186)",
         "wall distance q outside [0, 1] (particle_bounceback.F90:128-138)",
         "no extrapolation direction for an uncovered node (non-finite sphere state)",
-        "boundary-link list of a sphere overflowed"};
+        "boundary-link list of a sphere overflowed", "list of uncovered nodes overflowed"};
     return lbm_fail(LBM_ERR_PARTICLE, "particle run stopped at step %d: %s", err[1],
-                    err[0] >= 1 && err[0] <= 5 ? why[err[0]] : "unknown stop code");
+                    err[0] >= 1 && err[0] <= 6 ? why[err[0]] : "unknown stop code");
 }
 
 extern "C" {
@@ -973,6 +1016,17 @@ int lbm_particles_init(lbm_ctx* c, const lbm_particle_params* pp, const double* 
     if (e == hipSuccess) e = hipMalloc(&s->scal, sizeof(double) * 2);
     if (e == hipSuccess) e = hipMalloc(&s->red, sizeof(double) * 2 * RED_BLOCKS);
     if (e == hipSuccess) e = hipMalloc(&s->err, 2 * sizeof(int));
+    // uncovered nodes lie inside the old spheres: at most the nodes of all balls
+    {
+        double cap = 64.0;
+        for (int q = 0; q < pp->np; ++q) {
+            const double rb = radii[q] + 1.0;
+            cap += 4.0 / 3.0 * PI * rb * rb * rb;
+        }
+        s->refill_cap = (int)std::min(cap, 1.0e9);
+    }
+    if (e == hipSuccess) e = hipMalloc(&s->refill, sizeof(int) * 2 * (size_t)s->refill_cap);
+    if (e == hipSuccess) e = hipMalloc(&s->refill_count, sizeof(int));
     if (e == hipSuccess) e = hipMemcpy(s->xc[0], centers, v3, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->r, radii, sizeof(double) * pp->np, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = U ? hipMemcpy(s->U[0], U, v3, hipMemcpyHostToDevice) : hipMemset(s->U[0], 0, v3);
