@@ -199,14 +199,24 @@ __device__ __forceinline__ double wave_sum(double x) {
 }
 
 constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_CHUNK = 2048;  // bounding-box nodes tested per block round
 
 // Links (fluid node x, direction a with x + e_a owned by the sphere) of one
 // sphere's shell r < |x - c| <= r + 1.5 (the map's own distance test), in a
-// fixed order -- thread, node, direction -- into its slab.  Thread t walks
-// the bounding-box nodes t, t + 256, ... four at a time (independent loads in
-// flight); two walks: count, block prefix sum, write.
+// fixed order into its slab.  The bounding box is walked in chunks of
+// SCAN_CHUNK nodes (x fastest): the distance test alone picks the chunk's shell
+// nodes, which are compacted in LDS in box order; then every lane takes one
+// shell node and loads the owner words of the node and its 18 neighbours in
+// one go, so the gathers run with full waves (about one box node in seven is
+// in the shell).  Entries are written in one pass at a block-wide running
+// offset: per round of 256 shell nodes the waves in order, inside a wave
+// direction by direction, lane by lane -- 64 consecutive entries share a
+// direction and run along x, so the link pass's gathers from the same
+// population array fall on few cache lines.
 __global__ __launch_bounds__(SCAN_THREADS) void k_link_scan(const PArgs a) {
     __shared__ int halt;  // read once, so the whole block leaves together (barriers follow)
+    __shared__ unsigned shell[SCAN_CHUNK];
+    __shared__ int part[SCAN_CHUNK / SCAN_THREADS][SCAN_THREADS / 64];
     __shared__ int wave_tot[SCAN_THREADS / 64];
     if (threadIdx.x == 0) halt = stopped(a);
     __syncthreads();
@@ -221,86 +231,101 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_link_scan(const PArgs a) {
     unsigned* list = a.items + a.slab_off[id];
     const int cap = a.slab_cap[id];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    constexpr int UNROLL = 4;
-    // link mask of box node t (0 when t is past the box or not a shell fluid
-    // node).  The node's and its 18 neighbours' owner words are loaded in one
-    // go (indices clamped into the box, out-of-box neighbours masked after),
-    // so a node costs one round trip to memory, not one per direction.
-    auto links_at = [&](unsigned t, int& i, int& j, int& k) -> unsigned {
-        const unsigned ti = t % bx, tj = (t / bx) % by, tk = t / (bx * by);
-        i = lo[0] + (int)ti;
-        j = lo[1] + (int)tj;
-        k = lo[2] + (int)tk;
-        if (t >= total) return 0u;
-        const double d2 = dist2(i + 1.0 - cx, j + 1.0 - cy, k + 1.0 - cz);
-        if (!(d2 > rin2 && d2 <= rout2)) return 0u;
-        const int own = a.owner[i + (long long)a.nx * (j + (long long)a.ny * k)];
-        int nb[Q];
-        sfor<Q - 1>([&](auto B) {
-            constexpr int al = decltype(B)::value + 1;
-            const int ni = min(max(i + EX[al], 0), a.nx - 1), nj = min(max(j + EY[al], 0), a.ny - 1),
-                      nk = min(max(k + EZ[al], 0), a.nz - 1);
-            nb[al] = a.owner[ni + (long long)a.nx * (nj + (long long)a.ny * nk)];
-        });
-        if (own >= 0) return 0u;
-        unsigned mk = 0;
-        sfor<Q - 1>([&](auto B) {
-            constexpr int al = decltype(B)::value + 1;
-            if (in_box(a, i + EX[al], j + EY[al], k + EZ[al]) && nb[al] == id) mk |= 1u << al;
-        });
-        return mk;
+    constexpr int PER = SCAN_CHUNK / SCAN_THREADS;
+    auto coords = [&](unsigned t, int& i, int& j, int& k) {
+        i = lo[0] + (int)(t % bx);
+        j = lo[1] + (int)((t / bx) % by);
+        k = lo[2] + (int)(t / (bx * by));
     };
-    int cnt = 0;
-    for (unsigned base = 0; base < total; base += SCAN_THREADS * UNROLL) {
-        unsigned mk[UNROLL];
+    int pos = 0;  // entries written so far (block-uniform)
+    for (unsigned base = 0; base < total; base += SCAN_CHUNK) {
+        // 1. shell nodes of the chunk, compacted in box order: node
+        //    base + u * 256 + tid goes after every (u' < u) and every earlier
+        //    wave of the same u
+        bool in[PER];
+        unsigned long long bal[PER];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-            int i, j, k;
-            mk[u] = links_at(base + u * SCAN_THREADS + threadIdx.x, i, j, k);
+        for (int u = 0; u < PER; ++u) {
+            const unsigned t = base + u * SCAN_THREADS + threadIdx.x;
+            in[u] = false;
+            if (t < total) {
+                int i, j, k;
+                coords(t, i, j, k);
+                const double d2 = dist2(i + 1.0 - cx, j + 1.0 - cy, k + 1.0 - cz);
+                in[u] = d2 > rin2 && d2 <= rout2;
+            }
+            bal[u] = __ballot(in[u]);
+            if (lane == 0) part[u][wid] = (int)__popcll(bal[u]);
         }
+        __syncthreads();
+        int nshell = 0;
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) cnt += __popc(mk[u]);
-    }
-    // each wave owns a contiguous part of the slab (waves in order); inside
-    // it the entries go pass by pass, direction by direction, lane by lane:
-    // 64 consecutive entries share a direction and run along x, so the
-    // link pass's gathers from the same population array fall on few lines
-    int wcnt = cnt;
-    for (int off = 32; off > 0; off >>= 1) wcnt += __shfl_down(wcnt, off, 64);
-    if (lane == 0) wave_tot[wid] = wcnt;
-    __syncthreads();
-    int pos = 0, all = 0;
-    for (int w = 0; w < SCAN_THREADS / 64; ++w) {
-        if (w < wid) pos += wave_tot[w];
-        all += wave_tot[w];
-    }
-    if (all > cap) {  // cannot happen for a slab sized by the shell bound
-        if (threadIdx.x == 0) {
-            stop(a, 5);
-            a.link_count[id] = 0;
+        for (int u = 0; u < PER; ++u) {
+            int before = nshell;
+            for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+                if (w < wid) before += part[u][w];
+                nshell += part[u][w];
+            }
+            if (in[u]) {
+                const int within = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal[u] >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((unsigned)bal[u], 0u));
+                shell[before + within] = base + u * SCAN_THREADS + threadIdx.x;
+            }
         }
-        return;
-    }
-    for (unsigned base = 0; base < total; base += SCAN_THREADS * UNROLL) {
-        unsigned mk[UNROLL];
-        int ii[UNROLL], jj[UNROLL], kk[UNROLL];
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) mk[u] = links_at(base + u * SCAN_THREADS + threadIdx.x, ii[u], jj[u], kk[u]);
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-            const unsigned e0 = link_entry(ii[u] - lo[0], jj[u] - lo[1], kk[u] - lo[2], 0);
+        __syncthreads();
+        // 2. one lane per shell node: its links, then the entries
+        for (int r0 = 0; r0 < nshell; r0 += SCAN_THREADS) {
+            const int q = r0 + (int)threadIdx.x;
+            unsigned mk = 0u, e0 = 0u;
+            if (q < nshell) {
+                int i, j, k;
+                coords(shell[q], i, j, k);
+                e0 = link_entry(i - lo[0], j - lo[1], k - lo[2], 0);
+                const int own = a.owner[i + (long long)a.nx * (j + (long long)a.ny * k)];
+                int nb[Q];
+                sfor<Q - 1>([&](auto B) {
+                    constexpr int al = decltype(B)::value + 1;
+                    const int ni = min(max(i + EX[al], 0), a.nx - 1), nj = min(max(j + EY[al], 0), a.ny - 1),
+                              nk = min(max(k + EZ[al], 0), a.nz - 1);
+                    nb[al] = a.owner[ni + (long long)a.nx * (nj + (long long)a.ny * nk)];
+                });
+                if (own < 0) {
+                    sfor<Q - 1>([&](auto B) {
+                        constexpr int al = decltype(B)::value + 1;
+                        if (in_box(a, i + EX[al], j + EY[al], k + EZ[al]) && nb[al] == id) mk |= 1u << al;
+                    });
+                }
+            }
+            int wcnt = __popc(mk);
+            for (int off = 32; off > 0; off >>= 1) wcnt += __shfl_xor(wcnt, off, 64);
+            if (lane == 0) wave_tot[wid] = wcnt;
+            __syncthreads();
+            int at = pos, all = 0;
+            for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+                if (w < wid) at += wave_tot[w];
+                all += wave_tot[w];
+            }
             sfor<Q - 1>([&](auto B) {
                 constexpr int al = decltype(B)::value + 1;
-                const bool has = (mk[u] >> al) & 1u;
-                const unsigned long long bal = __ballot(has);
-                const int before = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                                  __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-                if (has) list[pos + before] = e0 | (unsigned)al;
-                pos += (int)__popcll(bal);
+                const bool has = (mk >> al) & 1u;
+                const unsigned long long b = __ballot(has);
+                const int before = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+                if (has && at + before < cap) list[at + before] = e0 | (unsigned)al;
+                at += (int)__popcll(b);
             });
+            pos += all;
+            __syncthreads();  // wave_tot is rewritten by the next round
         }
     }
-    if (threadIdx.x == 0) a.link_count[id] = all;
+    if (threadIdx.x == 0) {
+        if (pos > cap) {  // cannot happen for a slab sized by the shell bound
+            stop(a, 5);
+            a.link_count[id] = 0;
+        } else {
+            a.link_count[id] = pos;
+        }
+    }
 }
 
 // wave_off[id] = waves (64 links each) of the spheres before id; wave_off[np]
