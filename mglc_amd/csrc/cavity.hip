@@ -1029,6 +1029,7 @@ int lbm_step_begin(lbm_ctx* c) {
     int rc = launch_shell(c, a, true, false);
     if (rc) return rc;
     c->in_step = true;
+    c->halo_fresh = false;  // the caller exchanges; a later looped-back step re-exchanges first
     return LBM_OK;
 }
 
@@ -1049,6 +1050,7 @@ int lbm_step_end(lbm_ctx* c) {
 
 int lbm_halo_unpack(lbm_ctx* c, const void* dev_recv) {
     if (!c || !dev_recv) return lbm_fail(LBM_ERR_ARG, "lbm_halo_unpack: null argument");
+    c->halo_fresh = false;  // not the looped-back halo of the current state any more
     return launch_unpack(c, dev_recv, 0);
 }
 

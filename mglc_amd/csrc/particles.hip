@@ -85,6 +85,7 @@ namespace {
 
 constexpr double PI = 3.14159265358979323846;
 constexpr int REFILL_BLOCKS = 128;  // k_refill_proc: lanes walk the list with a grid stride
+constexpr int FLUID_BLOCKS = 4096;  // k_fluid_sum: enough waves in flight to stream the map and rho
 
 struct PArgs {
     const void* src;
@@ -769,12 +770,27 @@ __global__ __launch_bounds__(256) void k_refill_proc(const PArgs a) {
 // Sum of rho and count over fluid nodes, then rhoAvg into out.
 __global__ __launch_bounds__(RED_THREADS) void k_fluid_sum(const double* __restrict__ rho, const int* __restrict__ owner,
                                                            long long n, double* __restrict__ partial) {
+    // four nodes per thread and pass: one 16-byte owner load, two 16-byte
+    // rho loads (both arrays start on hipMalloc's alignment); the n % 4 last
+    // nodes go to the first threads of block 0.  Fixed grid and order, so the
+    // sums are the same on every run.
     double s = 0.0, cnt = 0.0;
-    for (long long c = blockIdx.x * (long long)RED_THREADS + threadIdx.x; c < n; c += (long long)gridDim.x * RED_THREADS)
-        if (owner[c] < 0) {
-            s += rho[c];
-            cnt += 1.0;
-        }
+    const long long groups = n / 4;
+    const int4* o4 = reinterpret_cast<const int4*>(owner);
+    const double2* r2 = reinterpret_cast<const double2*>(rho);
+    for (long long g = blockIdx.x * (long long)RED_THREADS + threadIdx.x; g < groups;
+         g += (long long)gridDim.x * RED_THREADS) {
+        const int4 o = o4[g];
+        const double2 ra = r2[2 * g], rb = r2[2 * g + 1];
+        if (o.x < 0) { s += ra.x; cnt += 1.0; }
+        if (o.y < 0) { s += ra.y; cnt += 1.0; }
+        if (o.z < 0) { s += rb.x; cnt += 1.0; }
+        if (o.w < 0) { s += rb.y; cnt += 1.0; }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (unsigned)(n - 4 * groups)) {
+        const long long c = 4 * groups + threadIdx.x;
+        if (owner[c] < 0) { s += rho[c]; cnt += 1.0; }
+    }
     __shared__ double sh[2][RED_THREADS / 64];
     s = wave_sum(s);
     cnt = wave_sum(cnt);
@@ -885,8 +901,8 @@ dim3 cell_grid(const lbm_ctx* c) {
 
 void launch_fluid_mean(lbm_ctx* c, const int* owner, double* out) {
     ParticleState* s = c->part;
-    hipLaunchKernelGGL(k_fluid_sum, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->macro, owner, c->n, s->red);
-    hipLaunchKernelGGL(k_fluid_mean, dim3(1), dim3(RED_THREADS), 0, c->stream, s->red, RED_BLOCKS, out);
+    hipLaunchKernelGGL(k_fluid_sum, dim3(FLUID_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->macro, owner, c->n, s->red);
+    hipLaunchKernelGGL(k_fluid_mean, dim3(1), dim3(RED_THREADS), 0, c->stream, s->red, FLUID_BLOCKS, out);
 }
 
 template <class S, int COLL>
@@ -1039,7 +1055,7 @@ int lbm_particles_init(lbm_ctx* c, const lbm_particle_params* pp, const double* 
     if (e == hipSuccess) e = hipMemcpy(s->slab_cap, cap.data(), sizeof(int) * pp->np, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(s->link_count, 0, sizeof(int) * pp->np);
     if (e == hipSuccess) e = hipMalloc(&s->scal, sizeof(double) * 2);
-    if (e == hipSuccess) e = hipMalloc(&s->red, sizeof(double) * 2 * RED_BLOCKS);
+    if (e == hipSuccess) e = hipMalloc(&s->red, sizeof(double) * 2 * FLUID_BLOCKS);
     if (e == hipSuccess) e = hipMalloc(&s->err, 2 * sizeof(int));
     // uncovered nodes lie inside the old spheres: at most the nodes of all balls
     {

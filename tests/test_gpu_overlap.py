@@ -152,3 +152,23 @@ def test_split_loopback_restart_is_exact(storage):
         got = b.macro()
     for q in range(4):
         assert np.array_equal(got[q], want[q]), q
+
+
+def test_split_loopback_after_a_caller_driven_step():
+    """A step made with step_begin / step_end on a looped-back split context
+    leaves the halo to the caller; the next lbm_step exchanges it again from
+    the state, so the run equals an uninterrupted one."""
+    nx, ny, nz = 64, 20, 16
+    print("[gpu] split loopback after begin/end", flush=True)
+    kw = dict(tauf=0.6, U0=0.05, storage="f64", halo_faces=0x3C, halo_loopback=1, halo_overlap=1)
+    fields = shear(nx, ny, nz, seed=4)
+    with Cavity(CavityConfig(nx, ny, nz, **kw)) as a, Cavity(CavityConfig(nx, ny, nz, **kw)) as b:
+        a.load_fields(*fields)
+        b.load_fields(*fields)
+        a.step(5)
+        a.step_begin()
+        a.step_end()
+        a.step(4)
+        b.step(10)
+        for q, (x, y) in enumerate(zip(a.macro(), b.macro())):
+            assert np.array_equal(x, y), q
