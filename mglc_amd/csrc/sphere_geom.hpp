@@ -8,6 +8,12 @@
 
 namespace lbm {
 
+// The bisection's stop tolerance: the reference declares
+// `real(8), parameter :: epsRadius=1e-9` (particle_bounceback.F90:109, and
+// CaseIV_DKT.F90:559), a default-real literal, so its value is the single
+// precision 1e-9 widened to double: 9.99999971718068537e-10.
+constexpr double CALQ_EPS = (double)1e-9f;
+
 // Squared distance with each product rounded and the sum taken left to
 // right, as (x - xc)**2 + (y - yc)**2 + (z - zc)**2 of initial.F90:78-89 and
 // particle_update.F90:69-79 evaluates: the solid map built from it is exact.
@@ -18,7 +24,7 @@ __host__ __device__ inline double sphere_dist2(double dx, double dy, double dz) 
 
 // calQ (particle_bounceback.F90:101-141): wall fraction q of the link from
 // node X along e = (ex, ey, ez) into the sphere (cx, cy, cz, rad) by
-// bisection from q = 1/2, stopping when |d - rad| < 1e-9 (at most 200
+// bisection from q = 1/2, stopping when |d - rad| < CALQ_EPS (at most 200
 // halvings); (x0, y0, z0) receives the last point.  Returns false when q
 // leaves [0, 1].
 // Same halvings, same stop rule and same branches as the reference's loop
@@ -46,7 +52,7 @@ __host__ __device__ inline bool calq(double cx, double cy, double cz, double rad
         bool outside;
         if (fabs(g) <= band) {
             const double d = sqrt(sphere_dist2(px - cx, py - cy, pz - cz));
-            if (fabs(d - rad) < 1e-9) break;
+            if (fabs(d - rad) < CALQ_EPS) break;
             outside = d > rad;
         } else {
             outside = g > 0.0;

@@ -81,6 +81,11 @@ void oracle_particle_map(const oracle_particle_params *p, const double *xc, cons
     }
 }
 
+/* The bisection's stop tolerance: the reference's `real(8), parameter ::
+ * epsRadius=1e-9` (particle_bounceback.F90:109) is a default-real literal, so
+ * it holds single-precision 1e-9 widened to double, 9.99999971718068537e-10. */
+static const double CALQ_EPS = (double)1e-9f;
+
 /* calQ (particle_bounceback.F90:101-141): bisection for the wall fraction q
  * of the link from node x (1-based coordinates) along e_a into sphere q_id;
  * x0 receives the boundary point.  Returns 0, or -1 if q leaves [0, 1]. */
@@ -91,7 +96,7 @@ int oracle_calq(const double *xc, const double *r, int id, const double x[3], in
     double px = x[0] + qt * EX[a], py = x[1] + qt * EY[a], pz = x[2] + qt * EZ[a];
     for (int it = 0; it < 200; ++it) {
         double d = sqrt(dist2(px - cx, py - cy, pz - cz));
-        if (fabs(d - rad) < 1e-9) break;
+        if (fabs(d - rad) < CALQ_EPS) break;
         qt = qt / 2.0;
         if (d > rad) {
             px += qt * EX[a];

@@ -22,7 +22,7 @@ static bool calq_plain(double cx, double cy, double cz, double rad, double X, do
     double px = X + qt * ex, py = Y + qt * ey, pz = Z + qt * ez;
     for (int it = 0; it < 200; ++it) {
         const double d = sqrt(lbm::sphere_dist2(px - cx, py - cy, pz - cz));
-        if (fabs(d - rad) < 1e-9) break;
+        if (fabs(d - rad) < (double)1e-9f) break;  // the reference's epsRadius
         qt = qt / 2.0;
         if (d > rad) {
             px += qt * ex; py += qt * ey; pz += qt * ez; q += qt;
@@ -65,7 +65,7 @@ int main(int argc, char** argv) {
                             memcmp(&z1, &z2, 8))
                             ++mismatches;
                         const double d = sqrt(lbm::sphere_dist2(x2 - cx, y2 - cy, z2 - cz));
-                        if (fabs(d - rad) < 1e-9) ++band;
+                        if (fabs(d - rad) < (double)1e-9f) ++band;
                     }
                 }
     }
