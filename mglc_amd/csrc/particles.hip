@@ -129,12 +129,24 @@ using lbm::calq;
 
 __device__ __forceinline__ double dist2(double dx, double dy, double dz) { return sphere_dist2(dx, dy, dz); }
 
+// The spin part of the body velocity at a point, W = Omega x (x - c); the
+// body velocity is U + W (CaseIV_DKT.F90:509-510 forms temp1, temp2 the same
+// way for a disc spinning about z).
+__device__ __forceinline__ void body_spin(const double* xc, const double* Om, int id, double x, double y, double z,
+                                          double& wx, double& wy, double& wz) {
+    const double rx = x - xc[3 * id], ry = y - xc[3 * id + 1], rz = z - xc[3 * id + 2];
+    wx = Om[3 * id + 1] * rz - Om[3 * id + 2] * ry;
+    wy = Om[3 * id + 2] * rx - Om[3 * id] * rz;
+    wz = Om[3 * id] * ry - Om[3 * id + 1] * rx;
+}
+
 __device__ __forceinline__ void body_velocity(const double* xc, const double* U, const double* Om, int id, double x,
                                               double y, double z, double& ux, double& uy, double& uz) {
-    const double rx = x - xc[3 * id], ry = y - xc[3 * id + 1], rz = z - xc[3 * id + 2];
-    ux = U[3 * id] + Om[3 * id + 1] * rz - Om[3 * id + 2] * ry;
-    uy = U[3 * id + 1] + Om[3 * id + 2] * rx - Om[3 * id] * rz;
-    uz = U[3 * id + 2] + Om[3 * id] * ry - Om[3 * id + 1] * rx;
+    double wx, wy, wz;
+    body_spin(xc, Om, id, x, y, z, wx, wy, wz);
+    ux = U[3 * id] + wx;
+    uy = U[3 * id + 1] + wy;
+    uz = U[3 * id + 2] + wz;
 }
 
 // Record a stop: the first code raised in a step call wins, with its step.
@@ -419,9 +431,10 @@ __global__ __launch_bounds__(256) void k_link_proc(const PArgs a) {
             const double s1 = abs_pop<S>(src, a.ps, c1, near ? al : bd, a.inv_scale);
             const double s2 = abs_pop<S>(src, a.ps, near ? c2 : c, near ? al : bd, a.inv_scale);
             const bool fl1 = in1 && o1 < 0, fl2 = in2 && o2 < 0;
-            double uwx, uwy, uwz;
-            body_velocity(a.xc, a.U, a.Om, id, x0, y0, z0, uwx, uwy, uwz);
-            const double eu = EX[bd] * uwx + EY[bd] * uwy + EZ[bd] * uwz;
+            double wx, wy, wz;
+            body_spin(a.xc, a.Om, id, x0, y0, z0, wx, wy, wz);
+            const double ux = a.U[3 * id], uy = a.U[3 * id + 1], uz = a.U[3 * id + 2];
+            const double eu = EX[bd] * (ux + wx) + EY[bd] * (uy + wy) + EZ[bd] * (uz + wz);
             const double rho_avg = a.rho_avg[0];
             double val;
             if (near) {
@@ -437,9 +450,11 @@ __global__ __launch_bounds__(256) void k_link_proc(const PArgs a) {
             }
             const long long cs = c + EX[al] + (long long)EY[al] * a.px + (long long)EZ[al] * a.pxy;
             S::st(src_w + (long long)bd * a.ps + cs, S::SHIFT ? R(val - weight(bd)) : R(val), a.scale, 0u);
-            const double dFx = (EX[al] - uwx) * fa - (EX[bd] - uwx) * val;
-            const double dFy = (EY[al] - uwy) * fa - (EY[bd] - uwy) * val;
-            const double dFz = (EZ[al] - uwz) * fa - (EZ[bd] - uwz) * val;
+            // Wen et al. (JCP 2014) exchange, (e - U - W) grouped as the
+            // reference groups it (CaseIV_DKT.F90:712-713)
+            const double dFx = (EX[al] - ux - wx) * fa - (EX[bd] - ux - wx) * val;
+            const double dFy = (EY[al] - uy - wy) * fa - (EY[bd] - uy - wy) * val;
+            const double dFz = (EZ[al] - uz - wz) * fa - (EZ[bd] - uz - wz) * val;
             const double rx = x0 - cx, ry = y0 - cy, rz = z0 - cz;
             acc[0] = dFx;
             acc[1] = dFy;
@@ -562,7 +577,7 @@ __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
             stop(a, 1);
             continue;
         }
-        const double g = (dij - rs - a.threshold_particle) / a.threshold_particle;
+        const double g = (dij - rp - a.r[q] - a.threshold_particle) / a.threshold_particle;
         const double mag = scale_p * (g * g);
         fs[0] += mag * dx / dij;
         fs[1] += mag * dy / dij;
@@ -570,23 +585,26 @@ __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
     }
     for (int d = 0; d < 3; ++d) fs[d] = wave_sum(fs[d]);
     if (lane != 0) return;
+    // wall springs summed apart from the particle springs and added last, as
+    // the reference keeps Fwxij apart from Fxij (particle_force.F90:192-193)
     const double scale_w = V * (a.rho_solid - a.rho0) * a.gravity / a.stiff_wall;
     const double N[3] = {(double)a.nx, (double)a.ny, (double)a.nz};
+    double fw[3] = {0.0, 0.0, 0.0};
     for (int d = 0; d < 3; ++d) {
         const double lo = a.xc[3 * p + d] - rp - 1.0;
         const double hi = N[d] - a.xc[3 * p + d] - rp;
         if (lo < 0.0 || hi < 0.0) stop(a, 2);
         if (lo < a.threshold_wall) {
             const double g = (lo - a.threshold_wall) / a.threshold_wall;
-            fs[d] += scale_w * (g * g);
+            fw[d] += scale_w * (g * g);
         }
         if (hi < a.threshold_wall) {
             const double g = (hi - a.threshold_wall) / a.threshold_wall;
-            fs[d] -= scale_w * (g * g);
+            fw[d] -= scale_w * (g * g);
         }
     }
-    double Fn[3] = {a.F[3 * p] + fs[0], a.F[3 * p + 1] + fs[1],
-                    a.F[3 * p + 2] - (a.rho_solid - rho_avg) * V * a.gravity + fs[2]};
+    double Fn[3] = {a.F[3 * p] + fs[0] + fw[0], a.F[3 * p + 1] + fs[1] + fw[1],
+                    a.F[3 * p + 2] - (a.rho_solid - rho_avg) * V * a.gravity + fs[2] + fw[2]};
     const double mass = a.rho_solid * V, inertia = 0.4 * mass * rp * rp;
     for (int d = 0; d < 3; ++d) {
         const double acc = Fn[d] / mass;

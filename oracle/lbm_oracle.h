@@ -125,6 +125,18 @@ int oracle_particle_run(const oracle_particle_params *p, int nsteps, double *xc,
 /* Wall-clock seconds spent inside the step loop of the last run. */
 double oracle_last_loop_seconds(void);
 
+/* The reference's two-dimensional two-disc particle program
+ * (Micro_particles/fortran/case4/seq/CaseIV_DKT.F90) restated
+ * (particle2d_oracle.c).  After nsteps steps: scal[19] = itc, shiftTimes,
+ * rhoAvg, xCenter[2], yCenter[2], Uc[2], Vc[2], rationalOmega[2],
+ * wallTotalForceX[2], wallTotalForceY[2], totalTorque[2]; rho, u, v, obst
+ * (nx*ny, x fastest) and, if f is not NULL, f (9*nx*ny, population fastest).
+ * Returns 0, -1 (allocation) or the reference's stop: 1 interpenetration,
+ * 2 wall penetration, 3 q outside [0, 1], 4 no extrapolation direction,
+ * 5 window moving too fast, 6 a disc near the bottom, 7 no disc owns a node. */
+int oracle_dkt_dims(int *nx, int *ny);
+int oracle_dkt_run(int nsteps, double *scal, double *rho, double *u, double *v, double *obst, double *f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -310,3 +310,65 @@ def particle_run(p: ParticleParams, nsteps: int, xc, r, U=None, Om=None):
     rc = _particle_lib().oracle_particle_run(ctypes.byref(p), nsteps, _dp(xc), _dp(U), _dp(Om), _dp(r), _dp(F), _dp(T),
                                              _ip(owner), _dp(rho), _dp(u), _dp(v), _dp(w), ctypes.byref(ra))
     return dict(rc=rc, xc=xc, U=U, Om=Om, F=F, T=T, owner=owner, rho=rho, u=u, v=v, w=w, rho_avg=ra.value)
+
+
+# ---- the reference's two-dimensional two-disc program (CaseIV_DKT.F90) --------
+REF_DKT = os.path.join(HERE, "_ref", "ref_dkt")
+DKT_SCALARS = ("itc", "shift", "rho_avg", "xc", "yc", "uc", "vc", "omega", "fx", "fy", "torque")
+
+
+def _dkt_unpack(scal):
+    out = {"itc": int(scal[0]), "shift": int(scal[1]), "rho_avg": float(scal[2])}
+    for q, name in enumerate(DKT_SCALARS[3:]):
+        out[name] = np.array(scal[3 + 2 * q: 5 + 2 * q])
+    return out
+
+
+def dkt_dims():
+    nx, ny = ctypes.c_int(), ctypes.c_int()
+    lib().oracle_dkt_dims(ctypes.byref(nx), ctypes.byref(ny))
+    return nx.value, ny.value
+
+
+def dkt_run(nsteps: int, with_f: bool = False):
+    """The restatement of CaseIV_DKT.F90 after nsteps steps: a dict of the
+    disc state (see DKT_SCALARS) and the fields rho, u, v, obst (ny, nx) and,
+    with with_f, f (ny, nx, 9).  Raises on a stop of the reference's."""
+    L = lib()
+    if not getattr(L, "_dkt_ready", False):
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.oracle_dkt_run.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, dp]
+        L.oracle_dkt_run.restype = ctypes.c_int
+        L._dkt_ready = True
+    nx, ny = dkt_dims()
+    scal = np.zeros(19)
+    rho, u, v, ob = (np.zeros((ny, nx)) for _ in range(4))
+    f = np.zeros((ny, nx, 9)) if with_f else None
+    rc = L.oracle_dkt_run(int(nsteps), _dp(scal), _dp(rho), _dp(u), _dp(v), _dp(ob), _dp(f) if with_f else None)
+    if rc != 0:
+        raise RuntimeError(f"oracle_dkt_run stopped with code {rc}")
+    out = _dkt_unpack(scal)
+    out.update(rho=rho, u=u, v=v, obst=ob)
+    if with_f:
+        out["f"] = f
+    return out
+
+
+def run_reference_dkt(nsteps: int):
+    """Run oracle/_ref/ref_dkt (the reference's own CaseIV_DKT.F90
+    subroutines) for nsteps steps in a scratch directory and read its state
+    in the layout of dkt_run(with_f=True)."""
+    import tempfile
+    nx, ny = dkt_dims()
+    with tempfile.TemporaryDirectory() as d:
+        subprocess.run([REF_DKT, str(int(nsteps))], cwd=d, check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL)
+        raw = np.fromfile(os.path.join(d, "dkt_state.bin"), dtype=np.float64)
+    n = nx * ny
+    assert raw.size == 19 + 4 * n + 9 * n, raw.size
+    out = _dkt_unpack(raw[:19])
+    body = raw[19:]
+    for q, name in enumerate(("rho", "u", "v", "obst")):
+        out[name] = body[q * n:(q + 1) * n].reshape(ny, nx)
+    out["f"] = body[4 * n:].reshape(ny, nx, 9)
+    return out

@@ -6,12 +6,18 @@
  *
  * TEST INFRASTRUCTURE ONLY (parity checker for the GPU particle path).
  *
- * The reference exists only in 2-D and its particle solver needs MPI, so it
- * is not run here; this restatement is pinned by closed forms (the particle
- * map against brute-force enumeration, the bisection wall distance against
- * the reference's own closed form calQ2, interpolated bounce-back reducing to
- * halfway bounce-back at q = 1/2, rest-state invariance).  Parity unpinned
- * against reference output for the 3-D generalisation.
+ * The reference's particle scheme exists only in 2-D.  Its per-link and
+ * per-particle formulas (calQ, the interpolated bounce-back with the moving
+ * wall, the momentum exchange, the springs, the refill extrapolation, the body
+ * velocity) are computed here by the functions of particle_scheme.h, which
+ * the 2-D restatement of the reference's sequential program
+ * (particle2d_oracle.c) also uses and which that restatement checks bit for
+ * bit against states the reference program itself wrote.  The 3-D assembly
+ * around them is pinned by closed forms (the particle map against
+ * brute-force enumeration, the bisection wall distance against the
+ * reference's own closed form calQ2, interpolated bounce-back reducing to
+ * halfway bounce-back at q = 1/2, rest-state invariance); the 3-D
+ * generalisation choices below have no reference output to compare with.
  *
  * Step (main.F90:35-69): collision on fluid nodes -> masked pull streaming ->
  * box-wall bounce-back -> interpolated bounce-back on particle links with the
@@ -37,6 +43,7 @@
 #include <string.h>
 
 #include "lbm_oracle.h"
+#include "particle_scheme.h"
 
 static const int EX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
 static const int EY[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
@@ -81,40 +88,14 @@ void oracle_particle_map(const oracle_particle_params *p, const double *xc, cons
     }
 }
 
-/* The bisection's stop tolerance: the reference's `real(8), parameter ::
- * epsRadius=1e-9` (particle_bounceback.F90:109) is a default-real literal, so
- * it holds single-precision 1e-9 widened to double, 9.99999971718068537e-10. */
-static const double CALQ_EPS = (double)1e-9f;
-
-/* calQ (particle_bounceback.F90:101-141): bisection for the wall fraction q
- * of the link from node x (1-based coordinates) along e_a into sphere q_id;
- * x0 receives the boundary point.  Returns 0, or -1 if q leaves [0, 1]. */
+/* calQ (particle_bounceback.F90:101-141) through the shared bisection: wall
+ * fraction q of the link from node x (1-based coordinates) along e_a into
+ * sphere q_id; x0 receives the boundary point.  Returns 0, or -1 if q leaves
+ * [0, 1]. */
 int oracle_calq(const double *xc, const double *r, int id, const double x[3], int a, double x0[3],
                 double *qout) {
-    const double cx = xc[3 * id], cy = xc[3 * id + 1], cz = xc[3 * id + 2], rad = r[id];
-    double q = 0.5, qt = 0.5;
-    double px = x[0] + qt * EX[a], py = x[1] + qt * EY[a], pz = x[2] + qt * EZ[a];
-    for (int it = 0; it < 200; ++it) {
-        double d = sqrt(dist2(px - cx, py - cy, pz - cz));
-        if (fabs(d - rad) < CALQ_EPS) break;
-        qt = qt / 2.0;
-        if (d > rad) {
-            px += qt * EX[a];
-            py += qt * EY[a];
-            pz += qt * EZ[a];
-            q += qt;
-        } else {
-            px -= qt * EX[a];
-            py -= qt * EY[a];
-            pz -= qt * EZ[a];
-            q -= qt;
-        }
-    }
-    x0[0] = px;
-    x0[1] = py;
-    x0[2] = pz;
-    *qout = q;
-    return (q > 1.0 || q < 0.0) ? -1 : 0;
+    const int e[3] = {EX[a], EY[a], EZ[a]};
+    return ps_calq(&xc[3 * id], r[id], x, e, x0, qout) ? -1 : 0;
 }
 
 /* calQ2 (particle_bounceback.F90:144-185): the closed form of the same q, the
@@ -180,13 +161,15 @@ static void collide_cell(const oracle_particle_params *p, const pbases *b, const
     }
 }
 
-/* Velocity of the rigid body `id` at point x: U + Omega x (x - c). */
-static void wall_velocity(const double *xc, const double *U, const double *Om, int id, const double x[3],
-                          double uw[3]) {
-    double rx = x[0] - xc[3 * id], ry = x[1] - xc[3 * id + 1], rz = x[2] - xc[3 * id + 2];
-    uw[0] = U[3 * id] + Om[3 * id + 1] * rz - Om[3 * id + 2] * ry;
-    uw[1] = U[3 * id + 1] + Om[3 * id + 2] * rx - Om[3 * id] * rz;
-    uw[2] = U[3 * id + 2] + Om[3 * id] * ry - Om[3 * id + 1] * rx;
+/* The motion of rigid body `id` seen at point x, split as the scheme uses
+ * it: the translation U and the spin part W = Omega x (x - c). */
+static void body_motion(const double *xc, const double *U, const double *Om, int id, const double x[3], double u[3],
+                        double w[3]) {
+    const double r[3] = {x[0] - xc[3 * id], x[1] - xc[3 * id + 1], x[2] - xc[3 * id + 2]};
+    u[0] = U[3 * id];
+    u[1] = U[3 * id + 1];
+    u[2] = U[3 * id + 2];
+    ps_spin(&Om[3 * id], r, w);
 }
 
 /* f_post of node (i,j,k) direction a for interpolation stencils: the rest
@@ -249,24 +232,19 @@ static int pstep(struct pstate *s) {
                         continue;
                     }
                     const int a = OPP[bdir]; /* link direction from x into the solid */
-                    double x[3] = {(double)(i + 1), (double)(j + 1), (double)(k + 1)}, x0[3], q, uw[3];
+                    double x[3] = {(double)(i + 1), (double)(j + 1), (double)(k + 1)}, x0[3], q, ub[3], wb[3];
                     if (oracle_calq(s->xc, s->r, id, x, a, x0, &q)) return -4;
-                    wall_velocity(s->xc, s->U, s->Om, id, x0, uw);
-                    double eu = EX[bdir] * uw[0] + EY[bdir] * uw[1] + EZ[bdir] * uw[2];
-                    double fa = s->fpost[19 * c + a];
-                    double val;
-                    if (q < 0.5) {
-                        double f1 = fpost_at(p, s->fpost, s->owner, i - EX[a], j - EY[a], k - EZ[a], a);
-                        double f2 = fpost_at(p, s->fpost, s->owner, i - 2 * EX[a], j - 2 * EY[a], k - 2 * EZ[a], a);
-                        val = q * (1.0 + 2.0 * q) * fa + (1.0 - 4.0 * q * q) * f1 - q * (1.0 - 2.0 * q) * f2 +
-                              6.0 * wgt(a) * s->rho_avg * eu;
-                    } else {
-                        double fb = s->fpost[19 * c + bdir];
-                        double fb1 = fpost_at(p, s->fpost, s->owner, i - EX[a], j - EY[a], k - EZ[a], bdir);
-                        val = fa / q / (1.0 + 2.0 * q) + fb * (2.0 * q - 1.0) / q -
-                              fb1 * (2.0 * q - 1.0) / (2.0 * q + 1.0) +
-                              6.0 * wgt(a) * s->rho_avg / q / (1.0 + 2.0 * q) * eu;
-                    }
+                    body_motion(s->xc, s->U, s->Om, id, x0, ub, wb);
+                    const int eb[3] = {EX[bdir], EY[bdir], EZ[bdir]};
+                    const double eu = ps_wall_dot(eb, ub, wb);
+                    const double fa = s->fpost[19 * c + a];
+                    /* q < 1/2 reads f_post(a) at x - e_a, x - 2 e_a; q >= 1/2
+                     * reads f_post(-a) at x and x - e_a */
+                    const double f1 = fpost_at(p, s->fpost, s->owner, i - EX[a], j - EY[a], k - EZ[a], a);
+                    const double f2 = fpost_at(p, s->fpost, s->owner, i - 2 * EX[a], j - 2 * EY[a], k - 2 * EZ[a], a);
+                    const double fb = s->fpost[19 * c + bdir];
+                    const double fb1 = fpost_at(p, s->fpost, s->owner, i - EX[a], j - EY[a], k - EZ[a], bdir);
+                    const double val = ps_interp_bb(q, fa, f1, f2, fb, fb1, wgt(a), s->rho_avg, eu);
                     s->f[19 * c + bdir] = val;
                 }
             }
@@ -298,14 +276,14 @@ static int pstep(struct pstate *s) {
                     if (!inside(p, ti, tj, tk)) continue;
                     int id = s->owner[PCELL(p, ti, tj, tk)];
                     if (id < 0) continue;
-                    double x[3] = {(double)(i + 1), (double)(j + 1), (double)(k + 1)}, x0[3], qq, uw[3];
+                    double x[3] = {(double)(i + 1), (double)(j + 1), (double)(k + 1)}, x0[3], qq, ub[3], wb[3];
                     if (oracle_calq(s->xc, s->r, id, x, a, x0, &qq)) return -4;
-                    wall_velocity(s->xc, s->U, s->Om, id, x0, uw);
+                    body_motion(s->xc, s->U, s->Om, id, x0, ub, wb);
                     const int b = OPP[a];
                     double fp = s->fpost[19 * c + a], fb = s->f[19 * c + b];
-                    double dF[3] = {(EX[a] - uw[0]) * fp - (EX[b] - uw[0]) * fb,
-                                    (EY[a] - uw[1]) * fp - (EY[b] - uw[1]) * fb,
-                                    (EZ[a] - uw[2]) * fp - (EZ[b] - uw[2]) * fb};
+                    double dF[3] = {ps_exchange(EX[a], EX[b], ub[0], wb[0], fp, fb),
+                                    ps_exchange(EY[a], EY[b], ub[1], wb[1], fp, fb),
+                                    ps_exchange(EZ[a], EZ[b], ub[2], wb[2], fp, fb)};
                     double rx = x0[0] - s->xc[3 * id], ry = x0[1] - s->xc[3 * id + 1], rz = x0[2] - s->xc[3 * id + 2];
                     s->F[3 * id] += dF[0];
                     s->F[3 * id + 1] += dF[1];
@@ -321,18 +299,19 @@ static int pstep(struct pstate *s) {
     double *Fn = (double *)malloc(sizeof(double) * 3 * np);
     for (int a = 0; a < np; ++a) {
         const double V = 4.0 / 3.0 * PI * s->r[a] * s->r[a] * s->r[a];
-        double fs[3] = {0, 0, 0};
+        /* particle and wall springs kept apart and added last, as Fxij and
+         * Fwxij are (particle_force.F90:192-193) */
+        double fs[3] = {0, 0, 0}, fw[3] = {0, 0, 0};
         const double scale_p = V * (p->rho_solid - s->rho_avg) * p->gravity / p->stiff_particle;
         for (int b2 = 0; b2 < np; ++b2) {
             if (b2 == a) continue;
             double dx = s->xc[3 * a] - s->xc[3 * b2], dy = s->xc[3 * a + 1] - s->xc[3 * b2 + 1],
                    dz = s->xc[3 * a + 2] - s->xc[3 * b2 + 2];
             double dij = sqrt(dist2(dx, dy, dz));
-            double rs = s->r[a] + s->r[b2];
-            if (dij >= rs + p->threshold_particle) continue;
-            if (dij < rs) { free(Fn); return -2; }
-            double g = (dij - rs - p->threshold_particle) / p->threshold_particle;
-            double mag = scale_p * (g * g);
+            if (dij >= s->r[a] + s->r[b2] + p->threshold_particle) continue;
+            if (dij < s->r[a] + s->r[b2]) { free(Fn); return -2; }
+            const double mag = ps_spring(scale_p, dij - s->r[a] - s->r[b2] - p->threshold_particle,
+                                         p->threshold_particle);
             fs[0] += mag * dx / dij;
             fs[1] += mag * dy / dij;
             fs[2] += mag * dz / dij;
@@ -342,18 +321,12 @@ static int pstep(struct pstate *s) {
             double lo = s->xc[3 * a + d] - s->r[a] - 1.0;
             double hi = N[d] - s->xc[3 * a + d] - s->r[a];
             if (lo < 0.0 || hi < 0.0) { free(Fn); return -3; }
-            if (lo < p->threshold_wall) {
-                double g = (lo - p->threshold_wall) / p->threshold_wall;
-                fs[d] += scale_w * (g * g);
-            }
-            if (hi < p->threshold_wall) {
-                double g = (hi - p->threshold_wall) / p->threshold_wall;
-                fs[d] -= scale_w * (g * g);
-            }
+            if (lo < p->threshold_wall) fw[d] += ps_spring(scale_w, lo - p->threshold_wall, p->threshold_wall);
+            if (hi < p->threshold_wall) fw[d] -= ps_spring(scale_w, hi - p->threshold_wall, p->threshold_wall);
         }
-        Fn[3 * a] = s->F[3 * a] + fs[0];
-        Fn[3 * a + 1] = s->F[3 * a + 1] + fs[1];
-        Fn[3 * a + 2] = s->F[3 * a + 2] - (p->rho_solid - s->rho_avg) * V * p->gravity + fs[2];
+        Fn[3 * a] = s->F[3 * a] + fs[0] + fw[0];
+        Fn[3 * a + 1] = s->F[3 * a + 1] + fs[1] + fw[1];
+        Fn[3 * a + 2] = s->F[3 * a + 2] - (p->rho_solid - s->rho_avg) * V * p->gravity + fs[2] + fw[2];
     }
     for (int a = 0; a < np; ++a) {
         const double V = 4.0 / 3.0 * PI * s->r[a] * s->r[a] * s->r[a];
@@ -414,7 +387,7 @@ static int pstep(struct pstate *s) {
                                  s->owner_new[PCELL(p, pi_, pj, pk)] < 0;
                         pt[m - 1] = ok ? s->f[19 * PCELL(p, pi_, pj, pk) + a] : wgt(a) * p->rho0;
                     }
-                    fr[a] = 3.0 * pt[0] - 3.0 * pt[1] + pt[2];
+                    fr[a] = ps_extrapolate(pt[0], pt[1], pt[2]);
                 }
                 /* replace the momentum moments with rhoAvg times the body velocity */
                 double m[19];
@@ -423,11 +396,11 @@ static int pstep(struct pstate *s) {
                     for (int a = 0; a < 19; ++a) acc += s->b.M[q][a] * fr[a];
                     m[q] = acc;
                 }
-                double xn[3] = {(double)(i + 1), (double)(j + 1), (double)(k + 1)}, ub[3];
-                wall_velocity(s->xc, s->U, s->Om, id, xn, ub);
-                m[3] = rho_avg2 * ub[0];
-                m[5] = rho_avg2 * ub[1];
-                m[7] = rho_avg2 * ub[2];
+                double xn[3] = {(double)(i + 1), (double)(j + 1), (double)(k + 1)}, ub[3], wb[3];
+                body_motion(s->xc, s->U, s->Om, id, xn, ub, wb);
+                m[3] = rho_avg2 * (ub[0] + wb[0]);
+                m[5] = rho_avg2 * (ub[1] + wb[1]);
+                m[7] = rho_avg2 * (ub[2] + wb[2]);
                 for (int a = 0; a < 19; ++a) {
                     double acc = 0.0;
                     for (int q = 0; q < 19; ++q) acc += s->b.Minv[a][q] * m[q];
