@@ -13,9 +13,10 @@ distance bisection, not by arithmetic: calQ stops at |d - r| < 1e-9
 halving apart, moving q by ~2^-30; the oracle itself moves its fields by
 ~5e-11 relative when a centre is perturbed by 1e-15 (tested on the CPU in
 test_oracle_particles.py).  FP32 storage is checked against the FP64 GPU path
-with the bound in the test.  The 3-D generalisation has no reference output
-(the reference's particle code is 2-D and needs MPI): parity unpinned
-against the reference, pinned against the oracle's closed-form checks."""
+with the bound in the test.  The reference's particle code is 2-D: the
+oracle's per-link formulas are those its 2-D restatement of CaseIV_DKT.F90
+reproduces bit for bit (test_oracle_dkt.py); the 3-D assembly around them
+has no reference output and is pinned by the oracle's closed-form checks."""
 import numpy as np
 import pytest
 

@@ -1,7 +1,8 @@
 """CPU tests of the moving-sphere oracle (D3Q19 generalisation of the
 reference's case4 particle coupling), pinned by brute force and closed forms.
-The 3-D generalisation has no reference output: parity unpinned against the
-reference itself (its particle code is 2-D and needs MPI)."""
+Its per-link formulas are shared with the 2-D restatement of the reference's
+CaseIV_DKT.F90, which reproduces that program bit for bit
+(test_oracle_dkt.py); the 3-D assembly has no reference output."""
 import numpy as np
 import pytest
 
