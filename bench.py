@@ -311,8 +311,8 @@ def bench_batch(args, stream_unused):
               "kernel": "k_batch_step (persistent, 100-step hipGraph replays)", "threads": threads,
               "blocks": blocks, "graph_steps": info.graph_steps}
     line = line_head(args, mlups, ms / args.steps, "f32", "synthetic (reference initial state per instance)", config)
-    # kernel time per launch: the graph holds one step kernel and one 1-thread
-    # step-counter kernel per step
+    # kernel time per launch: the graph holds one step kernel per step and
+    # one 1-thread step-counter kernel per 100 steps
     line["roofline"] = roofline(info.bytes_per_lup * cells, ms / args.steps, traffic_from_profiles("batch64 f32"))
     if args.attainable:
         # probed on one cube of about the batch's total cell count (the

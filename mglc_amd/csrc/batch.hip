@@ -50,6 +50,7 @@ struct BatchArgs {
     float scale, inv_scale;
     int meq12_rho;
     const unsigned* step_ctr;  // device step counter (seed of FP16 stochastic rounding)
+    unsigned step_add;         // this launch's step within its sequence: seed = *step_ctr + step_add
 };
 
 // Per-instance view with the members cavity_cell reads.
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(256) void k_batch_step(const BatchArgs b) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
-    const unsigned seed = *b.step_ctr;
+    const unsigned seed = *b.step_ctr + b.step_add;
     int m = 0;  // instance of the current tile; tiles only move forward, so m does too
     for (int t = wave; t < b.total_tiles; t += nwaves) {
         while (m + 1 < b.ninst && b.inst[m + 1].tile_begin <= t) ++m;
@@ -197,7 +198,7 @@ __global__ void k_batch_fill_initial(const BatchInst* __restrict__ inst, int nin
 }
 
 __global__ void k_counter_set(unsigned* ctr, unsigned v) { *ctr = v; }
-__global__ void k_counter_inc(unsigned* ctr) { *ctr = *ctr + 1u; }
+__global__ void k_counter_add(unsigned* ctr, unsigned n) { *ctr = *ctr + n; }
 
 using BatchStepFn = void (*)(const BatchArgs);
 using BatchInitFn = void (*)(const BatchArgs);
@@ -276,13 +277,20 @@ BatchInitFn batch_init_fn(const lbm_batch* b) {
     }
 }
 
-// One persistent step launch on b->stream (src = buf[cur]), then the counter.
-void launch_batch_step(lbm_batch* b, int cur, bool macro) {
+// One persistent step launch on b->stream (src = buf[cur]), step `step_add`
+// of a sequence; the sequence ends with advance_counter.
+void launch_batch_step(lbm_batch* b, int cur, bool macro, int step_add) {
     BatchArgs a = batch_args(b);
     a.src = b->buf[cur];
     a.dst = b->buf[1 - cur];
+    a.step_add = (unsigned)step_add;
     hipLaunchKernelGGL(batch_step_fn(b, macro), dim3(b->blocks), dim3(b->threads), 0, b->stream, a);
-    hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(1), 0, b->stream, b->ctr);
+}
+
+// The device step counter moves past a sequence of n steps (one launch per
+// sequence instead of one per step).
+void advance_counter(lbm_batch* b, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(1), 0, b->stream, b->ctr, (unsigned)n);
 }
 
 void drop_graph(lbm_batch* b) {
@@ -300,9 +308,10 @@ int capture_graph(lbm_batch* b, int cur) {
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal));
     int c = cur;
     for (int s = 0; s < GRAPH_STEPS; ++s) {
-        launch_batch_step(b, c, s == GRAPH_STEPS - 1);
+        launch_batch_step(b, c, s == GRAPH_STEPS - 1, s);
         c = 1 - c;
     }
+    advance_counter(b, GRAPH_STEPS);
     hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
     if (e != hipSuccess) {
         b->graph = nullptr;
@@ -487,9 +496,10 @@ int lbm_batch_step(lbm_batch* b, int nsteps) {
         remaining -= GRAPH_STEPS;  // even, so the current buffer is unchanged
     }
     for (int s = 0; s < remaining; ++s) {
-        launch_batch_step(b, b->cur, s == remaining - 1);
+        launch_batch_step(b, b->cur, s == remaining - 1, s);
         b->cur = 1 - b->cur;
     }
+    advance_counter(b, remaining);
     b->itc += nsteps;
     return lbm_check_launch();
 }
@@ -508,9 +518,9 @@ int lbm_batch_step_each(lbm_batch* b, int nsteps) {
             int rc = lbm_launch_cavity_step(a, b->p0.storage, b->p0.collision, macro, b->stream);
             if (rc) return rc;
         }
-        hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(1), 0, b->stream, b->ctr);
         b->cur = 1 - b->cur;
     }
+    advance_counter(b, nsteps);
     b->itc += nsteps;
     return lbm_check_launch();
 }
@@ -536,14 +546,15 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, int* best_threads, int* be
         for (int wi = 0; wi < 5; ++wi) {
             b->threads = threads_c[ti];
             b->blocks = std::max(1, cus * waves_per_cu_c[wi] / (threads_c[ti] / 64));
-            launch_batch_step(b, b->cur, false);  // warm-up
+            launch_batch_step(b, b->cur, false, 0);  // warm-up
             b->cur = 1 - b->cur;
             (void)hipEventRecord(e0, b->stream);
             for (int s = 0; s < trial_steps; ++s) {
-                launch_batch_step(b, b->cur, false);
+                launch_batch_step(b, b->cur, false, s + 1);
                 b->cur = 1 - b->cur;
             }
             (void)hipEventRecord(e1, b->stream);
+            advance_counter(b, trial_steps + 1);
             (void)hipEventSynchronize(e1);
             float ms = 0.0f;
             (void)hipEventElapsedTime(&ms, e0, e1);
