@@ -286,6 +286,37 @@ def random_spheres(nx: int, ny: int, nz: int, count: int = 2000, rmin: float = 4
     return xc, radii
 
 
+def jittered_grid_spheres(nx: int, ny: int, nz: int, count: int, radius: float = 10.0, pitch: float = 50.0,
+                          jitter: float = 20.0, seed: int = 1):
+    """Centres on a grid of `pitch` spacing, each moved by a uniform offset in
+    [-jitter/2, jitter/2) per axis, as the reference places its discs
+    (mpi_particle/initial.F90:48-84: start at pitch/2 = 25, step 50 along x,
+    and when x passes nx - 1 (200 for its 201-wide box) restart at 25 one
+    row up; offsets (rand - 0.5) * 20).  Carried to 3-D, a full row moves y
+    and a full layer moves z.  The offsets come from a seeded NumPy generator,
+    not the compiler's random_number (whose stream is its own), so the
+    layout, not the exact centres, is the reference's.  All spheres get
+    `radius` (radius = radius0 in the reference).  Returns (centres (count,
+    3), radii (count,)) in 1-based node coordinates."""
+    rng = np.random.default_rng(seed)
+    start = pitch / 2.0
+    x, y, z = start, start, start
+    xc = np.zeros((count, 3))
+    for q in range(count):
+        off = (rng.random(3) - 0.5) * jitter
+        xc[q] = (x + off[0], y + off[1], z + off[2])
+        x += pitch
+        if x > nx - 1:
+            x = start
+            y += pitch
+            if y > ny - 1:
+                y = start
+                z += pitch
+    if np.any(xc[:, 2] + radius > nz):
+        raise ValueError(f"{count} spheres at pitch {pitch} do not fit a {nx} x {ny} x {nz} box")
+    return xc, np.full(count, float(radius))
+
+
 def _relax_spheres(xc, r, N, min_gap, iters, step=0.6):
     """Push pairs closer than min_gap (and spheres closer than min_gap to a
     wall) apart, each by a fraction of its overlap, until none is left.

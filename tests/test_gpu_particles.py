@@ -20,7 +20,7 @@ has no reference output and is pinned by the oracle's closed-form checks."""
 import numpy as np
 import pytest
 
-from mglc_amd.lbm import Cavity, CavityConfig, LbmError, random_spheres
+from mglc_amd.lbm import Cavity, CavityConfig, LbmError, jittered_grid_spheres, random_spheres
 
 pytestmark = pytest.mark.gpu
 
@@ -104,6 +104,30 @@ def test_fp64_moving_spheres_match_oracle(oracle_lib):
     link_scale = 1.0 / 18.0  # largest f in a link term
     for name in ("F", "T"):
         assert np.abs(got[name] - ref[name]).max() <= 1e-10 * link_scale * r.max(), name
+
+
+def test_fp64_jittered_grid_with_springs_matches_oracle(oracle_lib):
+    """Eight spheres on the reference's jittered grid layout
+    (mpi_particle/initial.F90:48-84), packed so that particle-particle gaps
+    (2.3) and wall gaps (2.2) start inside the springs' threshold of 4:
+    both spring forces act from the first step."""
+    O = oracle_lib
+    n = (62, 36, 24)
+    xc, r = jittered_grid_spheres(*n, count=8, radius=5.0, pitch=14.0, jitter=4.0, seed=5)
+    xc = xc + 3.0
+    d = np.linalg.norm(xc[:, None] - xc[None], axis=2)
+    np.fill_diagonal(d, np.inf)
+    assert d.min() - 2 * 5.0 < 4.0 and (xc - r[:, None] - 1.0).min() < 4.0
+    steps, g = 80, 1e-4
+    print(f"[gpu] particles jittered grid n={n} steps={steps}", flush=True)
+    got = gpu_particles(box(*n), xc, r, steps, gravity=g)
+    ref = O.particle_run(O.particle_params(*n, 8, 0.8, gravity=g), steps, xc, r)
+    assert ref["rc"] == 0
+    assert np.array_equal(got["owner"], ref["owner"])
+    for name in ("rho", "u", "v", "w"):
+        assert relmax(got[name], ref[name]) <= FIELD_TOL, name
+    for name in ("xc", "U", "Om"):
+        assert relmax(got[name], ref[name]) <= 1e-12, name
 
 
 @pytest.mark.parametrize("collision", ["mrt", "bgk"])
