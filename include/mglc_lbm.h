@@ -80,9 +80,14 @@ typedef struct lbm_params {
     int halo_faces;       /* bit f (enum lbm_region, faces 0..5) set: face f borders a
                              neighbour subdomain and reads its halo; clear: wall
                              (halfway bounce-back; the z+ wall is the moving lid) */
-    int halo_loopback;    /* 1: before every step, pack all regions and unpack each
-                             into the opposite region of this same subdomain (the
-                             exchange looped back on one GPU) */
+    int halo_loopback;    /* 1: the exchange looped back on one GPU: every step also
+                             writes the populations leaving through each face into the
+                             opposite face's incoming region of this same subdomain, and
+                             each diagonal one leaving through an edge into that edge's
+                             region -- what a pack followed by an unpack into the
+                             opposite regions delivers -- for the regions the next step
+                             reads (all of them when halo_faces is 0: the exchange is
+                             then rehearsed, not read) */
     int thermal_axis;     /* thermal: axis of the walls held at t_hot (low side) and t_cold
                              (high side): 1 = y (LeftRightWallsConstT), 2 = z
                              (TopBottomPlatesConstT), 0 = x, -1 = none; other walls adiabatic */
@@ -142,9 +147,11 @@ int lbm_load_fields(lbm_ctx *ctx, const double *rho, const double *u, const doub
 /* nsteps iterations of collision -> streaming -> bounceback -> macro
  * (main.f90:87-97), fused into one pass over the lattice per step.  The
  * macroscopic fields of the last step are kept for lbm_check and
- * lbm_download_macro.  With halo_loopback the exchange runs before every
- * step; with halo faces and no loopback the caller exchanges between steps
- * (lbm_halo_pack / transport / lbm_halo_unpack), so nsteps must be 1. */
+ * lbm_download_macro.  With halo_loopback each step writes the next step's
+ * halo itself (the current state's halo is exchanged once first when it is
+ * not already in place); with halo faces and no loopback the caller
+ * exchanges between steps (lbm_halo_pack / transport / lbm_halo_unpack), so
+ * nsteps must be 1. */
 int lbm_step(lbm_ctx *ctx, int nsteps);
 
 /* One step in two halves, for a caller that overlaps its own exchange with the

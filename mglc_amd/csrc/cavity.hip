@@ -35,16 +35,80 @@ using namespace lbm;
 
 namespace {
 
+// The looped-back exchange folded into the step: every stored population
+// that leaves through a face is also written where the looped-back unpack
+// puts it in the next step's halo (a.halo_out) -- the incoming region of the
+// opposite face, same slot and plane index as k_halo_pack and k_halo_unpack
+// use, and for a diagonal population leaving through an edge the same edge
+// region -- for the regions in a.loop_mask (those the next step reads; all
+// of them when the exchange is only rehearsed).  The step then needs no pack
+// or unpack kernel.  (Across GPUs the same stores would go to the
+// neighbour's halo over the peer link.)
+// Only cells on the box's faces store anything: the whole hook is skipped
+// (one branch) elsewhere, which keeps the interior cells' instruction count
+// that of the plain step.
+template <class S>
+struct LoopbackHalo {
+    const StepArgs& a;
+    int i, j, k;
+    bool xlo, xhi, ylo, yhi, zlo, zhi;  // the cell lies on that face of the box
+    __device__ __forceinline__ LoopbackHalo(const StepArgs& args, int ci, int cj, int ck)
+        : a(args), i(ci), j(cj), k(ck), xlo(ci == 0), xhi(ci == args.nx - 1), ylo(cj == 0),
+          yhi(cj == args.ny - 1), zlo(ck == 0), zhi(ck == args.nz - 1) {}
+    __device__ __forceinline__ bool active() const { return xlo || xhi || ylo || yhi || zlo || zhi; }
+    template <int al, class T>
+    __device__ __forceinline__ void put(T t) const {
+        constexpr int ex = EX[al], ey = EY[al], ez = EZ[al];
+        T* out = static_cast<T*>(a.halo_out);
+        const unsigned m = a.loop_mask;
+        bool lx = false, ly = false, lz = false;
+        if constexpr (ex != 0) {
+            constexpr int leave = ex > 0 ? LBM_FACE_XP : LBM_FACE_XM, arrive = leave ^ 1;
+            constexpr int slot = cross_slot(arrive, al, -1);
+            static_assert(slot >= 0 && slot < 5, "a population leaving through a face has one of its 5 slots");
+            lx = ex > 0 ? xhi : xlo;
+            if (lx && ((m >> arrive) & 1u))
+                out[a.hoff[arrive] + (long long)slot * a.ny * a.nz + j + (long long)a.ny * k] = t;
+        }
+        if constexpr (ey != 0) {
+            constexpr int leave = ey > 0 ? LBM_FACE_YP : LBM_FACE_YM, arrive = leave ^ 1;
+            constexpr int slot = cross_slot(arrive, al, -1);
+            static_assert(slot >= 0 && slot < 5, "a population leaving through a face has one of its 5 slots");
+            ly = ey > 0 ? yhi : ylo;
+            if (ly && ((m >> arrive) & 1u))
+                out[a.hoff[arrive] + (long long)slot * a.nx * a.nz + i + (long long)a.nx * k] = t;
+        }
+        if constexpr (ez != 0) {
+            constexpr int leave = ez > 0 ? LBM_FACE_ZP : LBM_FACE_ZM, arrive = leave ^ 1;
+            constexpr int slot = cross_slot(arrive, al, -1);
+            static_assert(slot >= 0 && slot < 5, "a population leaving through a face has one of its 5 slots");
+            lz = ez > 0 ? zhi : zlo;
+            if (lz && ((m >> arrive) & 1u))
+                out[a.hoff[arrive] + (long long)slot * a.nx * a.ny + i + (long long)a.nx * j] = t;
+        }
+        if constexpr (al >= 7) {
+            // a diagonal population: on its edge when it leaves through both faces
+            constexpr int r = edge_region(al), fa = edge_free_axis(al);
+            const bool on = (ex == 0 || lx) && (ey == 0 || ly) && (ez == 0 || lz);
+            if (on && ((m >> r) & 1u)) out[a.hoff[r] + (fa == 0 ? i : (fa == 1 ? j : k))] = t;
+        }
+    }
+};
+
 // One lattice update per thread.  Block = (bx, by) with bx a multiple of 64:
 // one wavefront per x-row segment of 64 cells, so every population load and
 // store of a wave is 64 consecutive elements.
-template <class S, int COLL, bool MACRO, bool HALO>
+// LOOP: the exchange looped back inside the step (LoopbackHalo).
+template <class S, int COLL, bool MACRO, bool HALO, bool LOOP>
 __global__ __launch_bounds__(256) void k_cavity_step(const StepArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= a.nx || j >= a.ny) return;
-    cavity_cell<S, COLL, MACRO, HALO>(a, i, j, k);
+    if constexpr (LOOP)
+        cavity_cell<S, COLL, MACRO, HALO>(a, i, j, k, LoopbackHalo<S>{a, i, j, k});
+    else
+        cavity_cell<S, COLL, MACRO, HALO>(a, i, j, k);
 }
 
 // The split step (halo_overlap): the boundary shell -- every cell on a halo
@@ -59,40 +123,6 @@ struct ShellRows {
     int njf, jf[2];  // halo y faces: their rows j
     int k0, nzr;     // the k range [k0, k0 + nzr) not on a halo z face
     int rows;        // nzf * ny + njf * nzr
-};
-
-// The looped-back exchange folded into the shell: every stored population
-// that leaves through a halo face is also written where the looped-back
-// unpack puts it in the next step's halo (a.halo_out) -- the incoming region
-// of the opposite face, same slot and plane index as k_halo_pack and
-// k_halo_unpack use, and for the y-z edges the same edge region.  The step
-// then needs no pack or unpack kernel.  (Across GPUs the same stores would go
-// to the neighbour's halo over the peer link.)
-template <class S>
-struct LoopbackHalo {
-    const StepArgs& a;
-    int i, j, k;
-    template <int al, class T>
-    __device__ __forceinline__ void put(T t) const {
-        constexpr int ey = EY[al], ez = EZ[al];
-        T* out = static_cast<T*>(a.halo_out);
-        bool ly = false, lz = false;
-        if constexpr (ey != 0) {
-            constexpr int leave = ey > 0 ? LBM_FACE_YP : LBM_FACE_YM, arrive = leave ^ 1;
-            constexpr int slot = cross_slot(arrive, al, -1);
-            ly = (ey > 0 ? j == a.ny - 1 : j == 0) && ((a.halo_faces >> leave) & 1);
-            if (ly) out[a.hoff[arrive] + (long long)slot * a.nx * a.nz + i + (long long)a.nx * k] = t;
-        }
-        if constexpr (ez != 0) {
-            constexpr int leave = ez > 0 ? LBM_FACE_ZP : LBM_FACE_ZM, arrive = leave ^ 1;
-            constexpr int slot = cross_slot(arrive, al, -1);
-            lz = (ez > 0 ? k == a.nz - 1 : k == 0) && ((a.halo_faces >> leave) & 1);
-            if (lz) out[a.hoff[arrive] + (long long)slot * a.nx * a.ny + i + (long long)a.nx * j] = t;
-        }
-        if constexpr (ey != 0 && ez != 0) {
-            if (ly && lz) out[a.hoff[edge_region(al)] + i] = t;
-        }
-    }
 };
 
 template <class S, int COLL, bool MACRO, bool LOOP>
@@ -339,11 +369,16 @@ using InteriorFn = void (*)(const StepArgs, int, int, int);
 
 template <class S>
 struct Table {
-    static constexpr StepFn step[2][2][2] = {
-        {{k_cavity_step<S, COLL_MRT, false, false>, k_cavity_step<S, COLL_MRT, false, true>},
-         {k_cavity_step<S, COLL_MRT, true, false>, k_cavity_step<S, COLL_MRT, true, true>}},
-        {{k_cavity_step<S, COLL_BGK, false, false>, k_cavity_step<S, COLL_BGK, false, true>},
-         {k_cavity_step<S, COLL_BGK, true, false>, k_cavity_step<S, COLL_BGK, true, true>}}};
+    // step[collision][macro][halo][loop]
+    static constexpr StepFn step[2][2][2][2] = {
+        {{{k_cavity_step<S, COLL_MRT, false, false, false>, k_cavity_step<S, COLL_MRT, false, false, true>},
+          {k_cavity_step<S, COLL_MRT, false, true, false>, k_cavity_step<S, COLL_MRT, false, true, true>}},
+         {{k_cavity_step<S, COLL_MRT, true, false, false>, k_cavity_step<S, COLL_MRT, true, false, true>},
+          {k_cavity_step<S, COLL_MRT, true, true, false>, k_cavity_step<S, COLL_MRT, true, true, true>}}},
+        {{{k_cavity_step<S, COLL_BGK, false, false, false>, k_cavity_step<S, COLL_BGK, false, false, true>},
+          {k_cavity_step<S, COLL_BGK, false, true, false>, k_cavity_step<S, COLL_BGK, false, true, true>}},
+         {{k_cavity_step<S, COLL_BGK, true, false, false>, k_cavity_step<S, COLL_BGK, true, false, true>},
+          {k_cavity_step<S, COLL_BGK, true, true, false>, k_cavity_step<S, COLL_BGK, true, true, true>}}}};
     static constexpr InitFn init[2] = {k_cavity_init<S, COLL_MRT>, k_cavity_init<S, COLL_BGK>};
     static constexpr ShellFn shell[2][2][2] = {
         {{k_cavity_shell<S, COLL_MRT, false, false>, k_cavity_shell<S, COLL_MRT, false, true>},
@@ -359,6 +394,8 @@ struct Table {
 
 
 namespace {
+
+void loopback_masks(const lbm_ctx* c, unsigned* pack, unsigned* unpack);
 
 StepArgs make_args(const lbm_ctx* c) {
     StepArgs a{};
@@ -385,6 +422,8 @@ StepArgs make_args(const lbm_ctx* c) {
         a.hoff[r] = c->hoff[r];
         a.hcount[r] = c->hcount[r];
     }
+    unsigned pack_mask;
+    loopback_masks(c, &pack_mask, &a.loop_mask);
     return a;
 }
 
@@ -393,13 +432,13 @@ dim3 grid_of(const lbm_ctx* c) {
                 c->p.nz);
 }
 
-StepFn step_fn(const lbm_ctx* c, bool macro) {
+StepFn step_fn(const lbm_ctx* c, bool macro, bool loop) {
     const int coll = c->p.collision == LBM_BGK ? 1 : 0;
     const int halo = c->p.halo_faces != 0 ? 1 : 0;
     switch (c->p.storage) {
-    case LBM_STORE_F32: return Table<StoreF32>::step[coll][macro][halo];
-    case LBM_STORE_F16: return Table<StoreF16>::step[coll][macro][halo];
-    default: return Table<StoreF64>::step[coll][macro][halo];
+    case LBM_STORE_F32: return Table<StoreF32>::step[coll][macro][halo][loop];
+    case LBM_STORE_F16: return Table<StoreF16>::step[coll][macro][halo][loop];
+    default: return Table<StoreF64>::step[coll][macro][halo][loop];
     }
 }
 
@@ -531,21 +570,37 @@ int launch_interior(lbm_ctx* c, const StepArgs& a, bool macro) {
     return lbm_check_launch();
 }
 
+// With the loopback the step writes the next step's halo itself; a halo
+// that does not belong to the current state (after lbm_initial,
+// lbm_load_fields, lbm_load_state, a caller-driven step or unpack) is
+// exchanged once first.
+int fresh_loopback_halo(lbm_ctx* c) {
+    if (c->halo_fresh) return LBM_OK;
+    unsigned pmask, umask;
+    loopback_masks(c, &pmask, &umask);
+    int rc = launch_pack(c, c->buf[c->cur], c->hsend, pmask);
+    if (!rc) rc = launch_unpack(c, c->hsend, 1, umask);
+    if (rc) return rc;
+    c->halo_fresh = true;
+    return LBM_OK;
+}
+
+// After a looped-back step: the halo it wrote is the one the next step reads.
+void swap_halos(lbm_ctx* c) {
+    void* t = c->hin;
+    c->hin = c->hin_next;
+    c->hin_next = t;
+}
+
 // nsteps split steps: the shell, then the interior.  With the loopback the
 // shell also writes the next step's halo (LoopbackHalo) into the second halo
-// buffer, and the two buffers swap after each step; a halo that does not
-// belong to the current state (after lbm_initial, lbm_load_fields or
-// lbm_load_state) is exchanged once first.
+// buffer, and the two buffers swap after each step.
 int step_split(lbm_ctx* c, int nsteps) {
     const bool loop = c->p.halo_loopback != 0;
     int rc = LBM_OK;
-    if (loop && nsteps > 0 && !c->halo_fresh) {
-        unsigned pmask, umask;
-        loopback_masks(c, &pmask, &umask);
-        rc = launch_pack(c, c->buf[c->cur], c->hsend, pmask);
-        if (!rc) rc = launch_unpack(c, c->hsend, 1, umask);
+    if (loop && nsteps > 0) {
+        rc = fresh_loopback_halo(c);
         if (rc) return rc;
-        c->halo_fresh = true;
     }
     for (int s = 0; s < nsteps; ++s) {
         const bool macro = s == nsteps - 1;
@@ -558,11 +613,7 @@ int step_split(lbm_ctx* c, int nsteps) {
         if (!rc) rc = launch_interior(c, a, macro);
         if (rc) return rc;
         c->cur = 1 - c->cur;
-        if (loop) {
-            void* t = c->hin;
-            c->hin = c->hin_next;
-            c->hin_next = t;
-        }
+        if (loop) swap_halos(c);
     }
     return LBM_OK;
 }
@@ -600,9 +651,9 @@ int lbm_launch_cavity_step(const StepArgs& a, int storage, int collision, bool m
     const int coll = collision == LBM_BGK ? 1 : 0;
     StepFn fn;
     switch (storage) {
-    case LBM_STORE_F32: fn = Table<StoreF32>::step[coll][macro][0]; break;
-    case LBM_STORE_F16: fn = Table<StoreF16>::step[coll][macro][0]; break;
-    default: fn = Table<StoreF64>::step[coll][macro][0];
+    case LBM_STORE_F32: fn = Table<StoreF32>::step[coll][macro][0][0]; break;
+    case LBM_STORE_F16: fn = Table<StoreF16>::step[coll][macro][0][0]; break;
+    default: fn = Table<StoreF64>::step[coll][macro][0][0];
     }
     const int bx = a.nx >= 256 ? 256 : (a.nx + 63) / 64 * 64;
     const dim3 block(bx, 256 / bx, 1);
@@ -768,7 +819,7 @@ int lbm_create(const lbm_params* p, lbm_ctx** out) {
             return rc == LBM_ERR_HIP ? LBM_ERR_NOMEM : rc;
         }
     }
-    if (p->halo_overlap && p->halo_loopback) {
+    if (p->halo_loopback && !p->thermal) {
         if (hipMalloc(&c->hin_next, c->elem * c->htotal) != hipSuccess ||
             hipMemset(c->hin_next, 0, c->elem * c->htotal) != hipSuccess) {
             (void)hipGetLastError();
@@ -904,19 +955,20 @@ int lbm_step(lbm_ctx* c, int nsteps) {
     }
     StepArgs a = make_args(c);
     const dim3 grid = grid_of(c);
-    unsigned pmask, umask;
-    loopback_masks(c, &pmask, &umask);
+    const bool loop = c->p.halo_loopback != 0;
+    if (loop && nsteps > 0) {
+        int rc = fresh_loopback_halo(c);
+        if (rc) return rc;
+    }
     for (int s = 0; s < nsteps; ++s) {
-        if (c->p.halo_loopback) {
-            int rc = launch_pack(c, c->buf[c->cur], c->hsend, pmask);
-            if (!rc) rc = launch_unpack(c, c->hsend, 1, umask);
-            if (rc) return rc;
-        }
         a.src = c->buf[c->cur];
         a.dst = c->buf[1 - c->cur];
         a.seed = (unsigned)(c->itc + s + 1);
-        hipLaunchKernelGGL(step_fn(c, s == nsteps - 1), grid, c->block, 0, c->stream, a);
+        a.halo = c->hin;
+        a.halo_out = c->hin_next;
+        hipLaunchKernelGGL(step_fn(c, s == nsteps - 1, loop), grid, c->block, 0, c->stream, a);
         c->cur = 1 - c->cur;
+        if (loop) swap_halos(c);
     }
     c->itc += nsteps;
     return lbm_check_launch();

@@ -33,11 +33,14 @@ struct StepArgs {
     long long hoff[NREGION];    // element offset of each region in the packed layout
     long long hcount[NREGION];  // elements in each region
     unsigned seed;              // step number: drives the stochastic rounding of FP16 stores
-    void* halo_out;             // split step with loopback: the halo of the next step (shell kernel)
+    void* halo_out;             // loopback: the halo of the next step, written by the step itself
+    unsigned loop_mask;         // loopback: the regions of halo_out the next step reads (bit = region)
 };
 
-// No extra use of a cell's stored populations.
+// No extra use of a cell's stored populations.  A hook's active() says
+// whether its put() has anything to do for this cell at all.
 struct NoPost {
+    __device__ __forceinline__ static constexpr bool active() { return false; }
     template <int al, class T>
     __device__ __forceinline__ void put(T) const {}
 };
@@ -169,10 +172,12 @@ __device__ __forceinline__ void cavity_cell(const Args& a, int i, int j, int k, 
         stored[al] = S::cvt(f[al], a.scale, store_random<S, al>(h));
         __builtin_nontemporal_store(stored[al], dst + (long long)al * a.ps + c);
     });
-    sfor<Q>([&](auto A) {
-        constexpr int al = decltype(A)::value;
-        post.template put<al>(stored[al]);
-    });
+    if (post.active()) {
+        sfor<Q>([&](auto A) {
+            constexpr int al = decltype(A)::value;
+            post.template put<al>(stored[al]);
+        });
+    }
 }
 
 
