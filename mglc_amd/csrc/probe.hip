@@ -9,7 +9,8 @@
 //              thread reads the 19 populations of its cell from their
 //              upstream neighbours (population planes laid out as in
 //              lbm_create: x padded to 16, whole 4 KiB pages plus a 1 KiB
-//              stagger per plane) and writes them to its own cell
+//              stagger per plane) and writes them to its own cell with
+//              nontemporal stores, as the step kernel does
 //   pattern 2: the buoyancy step's traffic (FP64 only): the D3Q19 pull, the
 //              D3Q7 pull of the temperature populations and one force plane
 //              read and rewritten in place (27 planes in, 27 out)
@@ -50,9 +51,10 @@ __global__ __launch_bounds__(256) void k_probe_pull(const T* __restrict__ src, T
         const long long off = (long long)EX[al] + (long long)EY[al] * px + (long long)EZ[al] * pxy;
         v[p] = src[p * ps + c - off];
     });
+    // nontemporal stores, as the step kernels store their populations
     sfor<NP>([&](auto A) {
         constexpr int p = decltype(A)::value;
-        dst[p * ps + c] = v[p];
+        __builtin_nontemporal_store(v[p], dst + p * ps + c);
     });
 }
 
