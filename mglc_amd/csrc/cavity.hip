@@ -44,9 +44,10 @@ namespace {
 // of them when the exchange is only rehearsed).  The step then needs no pack
 // or unpack kernel.  (Across GPUs the same stores would go to the
 // neighbour's halo over the peer link.)
-// Only cells on the box's faces store anything: the whole hook is skipped
-// (one branch) elsewhere, which keeps the interior cells' instruction count
-// that of the plain step.
+// Only cells on the box's faces store anything and the hook is skipped (one
+// branch) elsewhere.  The x faces are the one lane at each end of a row: it
+// writes its five leaving populations through a base pointer computed once;
+// the rows on y or z faces (whole waves) write theirs and the edges'.
 template <class S>
 struct LoopbackHalo {
     const StepArgs& a;
@@ -56,42 +57,61 @@ struct LoopbackHalo {
         : a(args), i(ci), j(cj), k(ck), xlo(ci == 0), xhi(ci == args.nx - 1), ylo(cj == 0),
           yhi(cj == args.ny - 1), zlo(ck == 0), zhi(ck == args.nz - 1) {}
     __device__ __forceinline__ bool active() const { return xlo || xhi || ylo || yhi || zlo || zhi; }
-    template <int al, class T>
-    __device__ __forceinline__ void put(T t) const {
-        constexpr int ex = EX[al], ey = EY[al], ez = EZ[al];
+
+    // the populations leaving through x face `leave` (sign SX), into the
+    // incoming region of the opposite face
+    template <int SX, class T>
+    __device__ __forceinline__ void x_face(T* out, const T (&v)[Q]) const {
+        constexpr int leave = SX > 0 ? LBM_FACE_XP : LBM_FACE_XM, arrive = leave ^ 1;
+        if (!((a.loop_mask >> arrive) & 1u)) return;
+        const long long plane = (long long)a.ny * a.nz;
+        T* b = out + a.hoff[arrive] + j + (long long)a.ny * k;
+        sfor<Q>([&](auto A) {
+            constexpr int al = decltype(A)::value;
+            if constexpr (EX[al] == SX) {
+                constexpr int slot = cross_slot(arrive, al, -1);
+                static_assert(slot >= 0 && slot < 5, "a population leaving through a face has one of its 5 slots");
+                b[slot * plane] = v[al];
+            }
+        });
+    }
+
+    template <class T>
+    __device__ __forceinline__ void store(const T (&v)[Q]) const {
         T* out = static_cast<T*>(a.halo_out);
+        if (xhi) x_face<+1>(out, v);
+        if (xlo) x_face<-1>(out, v);
+        if (!(ylo || yhi || zlo || zhi)) return;  // uniform over a wave: its row
         const unsigned m = a.loop_mask;
-        bool lx = false, ly = false, lz = false;
-        if constexpr (ex != 0) {
-            constexpr int leave = ex > 0 ? LBM_FACE_XP : LBM_FACE_XM, arrive = leave ^ 1;
-            constexpr int slot = cross_slot(arrive, al, -1);
-            static_assert(slot >= 0 && slot < 5, "a population leaving through a face has one of its 5 slots");
-            lx = ex > 0 ? xhi : xlo;
-            if (lx && ((m >> arrive) & 1u))
-                out[a.hoff[arrive] + (long long)slot * a.ny * a.nz + j + (long long)a.ny * k] = t;
-        }
-        if constexpr (ey != 0) {
-            constexpr int leave = ey > 0 ? LBM_FACE_YP : LBM_FACE_YM, arrive = leave ^ 1;
-            constexpr int slot = cross_slot(arrive, al, -1);
-            static_assert(slot >= 0 && slot < 5, "a population leaving through a face has one of its 5 slots");
-            ly = ey > 0 ? yhi : ylo;
-            if (ly && ((m >> arrive) & 1u))
-                out[a.hoff[arrive] + (long long)slot * a.nx * a.nz + i + (long long)a.nx * k] = t;
-        }
-        if constexpr (ez != 0) {
-            constexpr int leave = ez > 0 ? LBM_FACE_ZP : LBM_FACE_ZM, arrive = leave ^ 1;
-            constexpr int slot = cross_slot(arrive, al, -1);
-            static_assert(slot >= 0 && slot < 5, "a population leaving through a face has one of its 5 slots");
-            lz = ez > 0 ? zhi : zlo;
-            if (lz && ((m >> arrive) & 1u))
-                out[a.hoff[arrive] + (long long)slot * a.nx * a.ny + i + (long long)a.nx * j] = t;
-        }
-        if constexpr (al >= 7) {
-            // a diagonal population: on its edge when it leaves through both faces
-            constexpr int r = edge_region(al), fa = edge_free_axis(al);
-            const bool on = (ex == 0 || lx) && (ey == 0 || ly) && (ez == 0 || lz);
-            if (on && ((m >> r) & 1u)) out[a.hoff[r] + (fa == 0 ? i : (fa == 1 ? j : k))] = t;
-        }
+        sfor<Q>([&](auto A) {
+            constexpr int al = decltype(A)::value;
+            constexpr int ex = EX[al], ey = EY[al], ez = EZ[al];
+            const bool lx = ex > 0 ? xhi : (ex < 0 ? xlo : false);
+            bool ly = false, lz = false;
+            if constexpr (ey != 0) {
+                constexpr int leave = ey > 0 ? LBM_FACE_YP : LBM_FACE_YM, arrive = leave ^ 1;
+                constexpr int slot = cross_slot(arrive, al, -1);
+                static_assert(slot >= 0 && slot < 5, "a population leaving through a face has one of its 5 slots");
+                ly = ey > 0 ? yhi : ylo;
+                if (ly && ((m >> arrive) & 1u))
+                    out[a.hoff[arrive] + (long long)slot * a.nx * a.nz + i + (long long)a.nx * k] = v[al];
+            }
+            if constexpr (ez != 0) {
+                constexpr int leave = ez > 0 ? LBM_FACE_ZP : LBM_FACE_ZM, arrive = leave ^ 1;
+                constexpr int slot = cross_slot(arrive, al, -1);
+                static_assert(slot >= 0 && slot < 5, "a population leaving through a face has one of its 5 slots");
+                lz = ez > 0 ? zhi : zlo;
+                if (lz && ((m >> arrive) & 1u))
+                    out[a.hoff[arrive] + (long long)slot * a.nx * a.ny + i + (long long)a.nx * j] = v[al];
+            }
+            if constexpr (al >= 7) {
+                // a diagonal population: on its edge when it leaves through both
+                // faces (an edge always has a y or z component)
+                constexpr int r = edge_region(al), fa = edge_free_axis(al);
+                const bool on = (ex == 0 || lx) && (ey == 0 || ly) && (ez == 0 || lz);
+                if (on && ((m >> r) & 1u)) out[a.hoff[r] + (fa == 0 ? i : (fa == 1 ? j : k))] = v[al];
+            }
+        });
     }
 };
 

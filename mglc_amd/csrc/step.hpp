@@ -38,11 +38,12 @@ struct StepArgs {
 };
 
 // No extra use of a cell's stored populations.  A hook's active() says
-// whether its put() has anything to do for this cell at all.
+// whether its store() has anything to do for this cell at all; store() sees
+// the 19 values as stored.
 struct NoPost {
     __device__ __forceinline__ static constexpr bool active() { return false; }
-    template <int al, class T>
-    __device__ __forceinline__ void put(T) const {}
+    template <class T>
+    __device__ __forceinline__ void store(const T (&)[Q]) const {}
 };
 
 enum { COLL_MRT = 0, COLL_BGK = 1 };
@@ -118,7 +119,7 @@ __device__ __forceinline__ typename S::Real pull_outside(const Args& a, const ty
 
 // The update of cell (i, j, k): pull the 19 populations (bounce-back or halo
 // where the pull leaves the domain), macro, collision, store.  `post` sees
-// each stored value (post.put<a>(value)), e.g. to write it to a halo too,
+// the stored values (post.store(values)), e.g. to write them to a halo too,
 // after the last store: the collision and its stores stay one straight-line
 // block in every kernel, so the compiler contracts the arithmetic the same
 // way and every kernel built on this function produces the same bits.
@@ -172,12 +173,7 @@ __device__ __forceinline__ void cavity_cell(const Args& a, int i, int j, int k, 
         stored[al] = S::cvt(f[al], a.scale, store_random<S, al>(h));
         __builtin_nontemporal_store(stored[al], dst + (long long)al * a.ps + c);
     });
-    if (post.active()) {
-        sfor<Q>([&](auto A) {
-            constexpr int al = decltype(A)::value;
-            post.template put<al>(stored[al]);
-        });
-    }
+    if (post.active()) post.store(stored);
 }
 
 
