@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <new>
 #include <vector>
@@ -378,7 +379,7 @@ int lbm_batch_create(const lbm_params* params, int count, lbm_batch** out) {
     b->inv_scale = 1.0f / b->scale;
     b->host.resize(count);
     long long off = 0, moff = 0, loff = 0;
-    int tiles = 0;
+    long long tiles = 0;
     const long long page = 4096 / (long long)b->elem;
     for (int m = 0; m < count; ++m) {
         const lbm_params& p = params[m];
@@ -400,14 +401,18 @@ int lbm_batch_create(const lbm_params* params, int count, lbm_batch** out) {
         in.snu = 1.0 / p.tau;
         in.sq = 8.0 * (2.0 * p.tau - 1.0) / (8.0 * p.tau - 1.0);
         in.u_lid = p.u_lid;
-        in.tile_begin = tiles;
+        in.tile_begin = (int)tiles;
         in.pad = 0;
-        tiles += (int)((in.ncells + TILE - 1) / TILE);
+        tiles += (in.ncells + TILE - 1) / TILE;
+        if (tiles > INT_MAX) {
+            delete b;
+            return lbm_fail(LBM_ERR_ARG, "lbm_batch_create: more than %d tiles of 64 cells in the batch", INT_MAX);
+        }
     }
     b->total_elems = off;
     b->total_cells = moff;
     b->total_lid = loff;
-    b->total_tiles = tiles;
+    b->total_tiles = (int)tiles;
     hipError_t e = hipStreamCreate(&b->stream);
     for (auto& p : b->buf)
         if (e == hipSuccess) e = hipMalloc(&p, b->elem * (size_t)off);
