@@ -106,6 +106,18 @@ def test_bandwidth_probe_rejects_bad_arguments(hip_lib):
     assert b"lbm_bandwidth_probe" in hip_lib.lbm_last_error()
 
 
+def test_batch_create_rejects_more_tiles_than_int_range(hip_lib):
+    """300 instances of 1024 x 1024 x 512 cells are 2.5e9 tiles of 64 cells:
+    refused while the tile table is built, before any device allocation."""
+    from mglc_amd import lbm
+    one = lbm.CavityConfig(1024, 1024, 512, storage="f16").params()
+    arr = (lbm.LbmParams * 300)(*([one] * 300))
+    b = ctypes.c_void_p()
+    assert hip_lib.lbm_batch_create(arr, 300, ctypes.byref(b)) == 1
+    assert b"tiles" in hip_lib.lbm_last_error()
+    assert not b.value
+
+
 def test_create_validates_the_split_step(hip_lib):
     """halo_overlap: y/z halo faces only, isothermal, with halo faces, and a
     looped-back split needs both faces of each halo axis."""
