@@ -92,6 +92,18 @@ def test_residual_sequence_matches_oracle(oracle_lib):
     np.testing.assert_allclose(got, errs, rtol=1e-10)
 
 
+def test_residual_matches_the_reference_programs_known_answer():
+    """errorU at itc = 2000 of the reference program itself
+    (mpi_3d_blocked, 17^3, Re = 100, U0 = 0.1; check.f90:9-34), as a one-rank
+    run prints it; its 2-, 4- and 8-rank runs differ from it by up to 4e-16
+    relative."""
+    with Cavity(CavityConfig(17, 17, 17, Reynolds=100.0, U0=0.1)) as c:
+        c.initial()
+        c.step(2000)
+        e = c.check()
+    assert e == pytest.approx(0.86857284647299848, rel=1e-12)
+
+
 def test_mass_conserved_on_gpu():
     rho, *_ = gpu_run(CavityConfig(40, 36, 32, tauf=0.6), 400)
     assert abs(rho.sum() - rho.size) / rho.size < 1e-13

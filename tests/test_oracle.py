@@ -135,3 +135,22 @@ def test_check_closed_form(oracle_lib):
     assert np.all(up == 2.0)
     e2 = f(ctypes.byref(p), *(O._dp(a) for a in (u, v, w, up, vp)))
     assert e2 == 0.0
+
+
+# errorU at itc = 2000 of the reference program itself
+# (MPI/Lid_driven_cavity/fortran/3d/mpi_3d_blocked, N = 17, Re = 100, U0 = 0.1,
+# so tauf = 3 U0 N / Re + 0.5 = 0.551; check.f90:9-34 after 2000 steps of
+# main.f90:85-101), as printed by a one-rank run; runs on 2, 4 and 8 ranks
+# printed 0.86857284647299815, ...915 and ...893 (the Allreduce order).
+REFERENCE_ERROR_U_17_RE100_ITC2000 = 0.86857284647299848
+
+
+def test_residual_matches_the_reference_programs_known_answer(oracle_lib):
+    """The whole step chain and check over 2000 steps against the number the
+    reference program prints, to the spread of its own rank counts."""
+    O = oracle_lib
+    N = 17
+    p = O.cavity_params(N, N, N, 3 * 0.1 * N / 100 + 0.5, 0.1)
+    errs = O.cavity_run(p, 2000, check_every=2000)[-1]
+    assert len(errs) == 1
+    assert errs[0] == pytest.approx(REFERENCE_ERROR_U_17_RE100_ITC2000, rel=1e-14)
