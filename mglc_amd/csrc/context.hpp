@@ -44,6 +44,8 @@ struct lbm_ctx {
     long long itc = 0;
     bool initialised = false;
     dim3 block{64, 4, 1};
+    unsigned step_lds = 0;       // LDS bytes reserved per block of the step kernels: caps the
+                                 // blocks resident per CU (0: no cap); see lbm_create
     float scale = 1.0f, inv_scale = 1.0f;
     long long device_bytes = 0;
     long long hoff[lbm::NREGION] = {}, hcount[lbm::NREGION] = {}, htotal = 0;
