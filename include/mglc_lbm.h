@@ -327,9 +327,10 @@ int lbm_batch_get_info(const lbm_batch *batch, lbm_batch_info *info);
  * bandwidth (GB/s, read + write) the memory system delivers for the step's
  * traffic with no arithmetic: pattern 0 = a one-stream copy of 19 * n^3
  * elements, pattern 1 = the D3Q19 pull of an n^3 lattice in the library's
- * layout (19 upstream loads and 19 stores per cell), pattern 2 = the
- * buoyancy step's traffic (D3Q19 + D3Q7 pulls and one force plane, FP64:
- * 27 loads and 27 stores per cell).  storage = element size
+ * layout (19 upstream loads and 19 nontemporal stores per cell), pattern 2 =
+ * the buoyancy step's traffic (D3Q19 + D3Q7 pulls and one force plane, FP64:
+ * 27 loads and 27 stores per cell); patterns 1 and 2 report the best of four
+ * occupancies (all waves, 16, 12 and 8 waves per CU).  storage = element size
  * (LBM_STORE_*), 16 <= n <= 512, reps timed launches after 3 untimed ones.
  * Allocates and frees two lattices of its own. */
 int lbm_bandwidth_probe(int pattern, int storage, int n, int reps, double *gbps);
