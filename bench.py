@@ -184,8 +184,8 @@ def run_storage(args, storage, stream):
 def overlap_compare(args, stream):
     """The y and z faces looped back (a periodic y-z box: the exchange of a
     subdomain with four neighbours, on one GPU), timed with the fused step
-    (exchange between steps) and with the split step (boundary shell first,
-    its exchange on a second stream while the interior is updated)."""
+    (which writes the next step's halo itself) and with the split step
+    (boundary shell first, writing the halo, then the interior)."""
     from mglc_amd.lbm import Cavity, CavityConfig
 
     n = args.n
