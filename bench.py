@@ -57,10 +57,11 @@ def parse():
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle on a bounded sample")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--halo-loopback", type=int, default=1,
-                    help="cavity: 1 = pack/unpack the six faces and twelve edges every step, looped back")
+                    help="cavity: 1 = the six faces and twelve edges exchanged every step, looped back "
+                         "(written by the step kernel)")
     ap.add_argument("--overlap", type=int, default=1,
-                    help="cavity: also time y/z faces looped back, fused step vs split step with the exchange "
-                         "overlapped")
+                    help="cavity: also time y/z faces looped back, fused step vs split step (shell, then "
+                         "interior)")
     ap.add_argument("--attainable", type=int, default=1,
                     help="cavity / batch: also time the step's traffic with no arithmetic (lbm_bandwidth_probe)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -193,7 +194,7 @@ def overlap_compare(args, stream):
     steps = min(args.steps, 500)
     out = {"halo": "y and z faces looped back: 4 faces x 5 pops + 4 y-z edges per step", "storage": "f64",
            "steps": steps}
-    for name, ov in (("fused_serial", 0), ("split_overlapped", 1)):
+    for name, ov in (("fused", 0), ("split", 1)):
         with Cavity(CavityConfig(n, n, n, halo_overlap=ov, **base), stream=stream.handle.value) as cav:
             cav.initial()
             ms, _ = timed(cav.step, stream, argparse.Namespace(warmup=args.warmup, steps=steps))
