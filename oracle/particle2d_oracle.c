@@ -13,8 +13,8 @@
  * Every sum runs in the reference's order and every expression is evaluated
  * in its order (left to right, products rounded; x**2.0d0 is x*x as the
  * reference's -O3 build computes it), so with contraction off the results
- * are the reference's bits; the per-node loops that write only their own
- * node (collision, streaming, macro) run in parallel, which changes no value.
+ * are the reference's bits; the per-node loops whose writes never overlap
+ * (collision, push streaming, macro) run in parallel, which changes no value.
  * The per-link and per-particle formulas are the functions of
  * particle_scheme.h, shared with the three-dimensional oracle.  Parameters
  * are the reference's compile-time ones (CaseIV_DKT.F90:1-62).
