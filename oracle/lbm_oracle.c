@@ -375,6 +375,10 @@ static void set_threads(int nthreads) {
 #endif
 }
 
+/* Threads of the OpenMP loops that follow (the particle oracles' per-node
+ * loops); n <= 0 leaves the setting as it is. */
+void oracle_set_threads(int n) { set_threads(n); }
+
 int oracle_cavity_run(const oracle_cavity_params *p, int nsteps, int check_every,
                       double *rho, double *u, double *v, double *w,
                       double *errs, int nerr_max, int nthreads) {

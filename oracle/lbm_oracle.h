@@ -36,6 +36,10 @@ typedef struct {
  * check_every steps (0 = never).  Outputs rho,u,v,w after the last step; errs
  * (may be NULL) receives errorU at each check, nerr_max entries at most.
  * Returns the number of checks done, or -1 on allocation failure. */
+/* Threads of the OpenMP loops that follow (the particle oracles' per-node
+ * loops); n <= 0 leaves the setting as it is. */
+void oracle_set_threads(int n);
+
 int oracle_cavity_run(const oracle_cavity_params *p, int nsteps, int check_every,
                       double *rho, double *u, double *v, double *w,
                       double *errs, int nerr_max, int nthreads);

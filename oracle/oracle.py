@@ -294,6 +294,15 @@ def calq(xc, r, pid, x, a):
     return rc, q.value, x0, closed
 
 
+def set_threads(n: int) -> None:
+    """Threads of the oracle's OpenMP loops that take no thread count of
+    their own (the particle oracles' per-node loops)."""
+    L = lib()
+    L.oracle_set_threads.argtypes = [ctypes.c_int]
+    L.oracle_set_threads.restype = None
+    L.oracle_set_threads(int(n))
+
+
 def particle_run(p: ParticleParams, nsteps: int, xc, r, U=None, Om=None):
     """Spheres at xc (np x 3, 1-based node coordinates) with radii r, fluid at
     rest; returns a dict of the state after nsteps (rc != 0 on a stop)."""

@@ -208,13 +208,18 @@ static int pstep(struct pstate *s) {
     const oracle_particle_params *p = &s->p;
     const int nx = p->nx, ny = p->ny, nz = p->nz, np = p->np;
     const size_t n = (size_t)nx * ny * nz;
-    /* collision on fluid nodes (fluid.F90:1-72) */
-    for (size_t c = 0; c < n; ++c)
+    /* collision on fluid nodes (fluid.F90:1-72); nodes are independent, so
+     * the loop runs in parallel without changing any value */
+#pragma omp parallel for schedule(static)
+    for (long long c = 0; c < (long long)n; ++c)
         if (s->owner[c] < 0) collide_cell(p, &s->b, &s->f[19 * c], s->rho[c], s->u[c], s->v[c], s->w[c], &s->fpost[19 * c]);
     /* rhoAvg over fluid nodes of the last macro (particle_bounceback.F90:15-35) */
     s->rho_avg = fluid_mean_rho(s, s->owner);
     /* streaming + walls + particle bounce-back (fluid.F90:75-161,
-     * particle_bounceback.F90:37-96) */
+     * particle_bounceback.F90:37-96); each node writes only its own
+     * populations, so the loop runs in parallel */
+    int bad_q = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad_q)
     for (int k = 0; k < nz; ++k)
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
@@ -233,7 +238,10 @@ static int pstep(struct pstate *s) {
                     }
                     const int a = OPP[bdir]; /* link direction from x into the solid */
                     double x[3] = {(double)(i + 1), (double)(j + 1), (double)(k + 1)}, x0[3], q, ub[3], wb[3];
-                    if (oracle_calq(s->xc, s->r, id, x, a, x0, &q)) return -4;
+                    if (oracle_calq(s->xc, s->r, id, x, a, x0, &q)) {
+                        bad_q = 1;
+                        continue;
+                    }
                     body_motion(s->xc, s->U, s->Om, id, x0, ub, wb);
                     const int eb[3] = {EX[bdir], EY[bdir], EZ[bdir]};
                     const double eu = ps_wall_dot(eb, ub, wb);
@@ -248,8 +256,10 @@ static int pstep(struct pstate *s) {
                     s->f[19 * c + bdir] = val;
                 }
             }
+    if (bad_q) return -4;
     /* macro on fluid nodes (fluid.F90:164-184) */
-    for (size_t c = 0; c < n; ++c) {
+#pragma omp parallel for schedule(static)
+    for (long long c = 0; c < (long long)n; ++c) {
         if (s->owner[c] >= 0) continue;
         double r = 0, x = 0, y = 0, z = 0;
         for (int a = 0; a < 19; ++a) {

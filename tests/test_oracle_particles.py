@@ -121,3 +121,21 @@ def test_fields_are_conditioned_by_the_bisection_stop(oracle_lib):
     d = np.abs(a["u"] - b["u"]).max() / np.abs(a["u"]).max()
     assert 1e-13 < d < 1e-9
     assert np.abs(a["xc"] - b["xc"]).max() < 1e-12
+
+
+def test_parallel_per_node_loops_change_no_value(oracle_lib):
+    """The oracle's per-node loops (collision, streaming with the particle
+    bounce-back, macro) run in parallel: one thread and many give the same
+    bits in every field and every sphere quantity."""
+    from mglc_amd.lbm import random_spheres
+    O = oracle_lib
+    xc, r = random_spheres(48, 44, 40, count=12, rmin=3.0, rmax=6.0, seed=7)
+    p = O.particle_params(48, 44, 40, 12, 0.65, rho_solid=1.5, gravity=1e-4)
+    O.set_threads(1)
+    a = O.particle_run(p, 60, xc, r)
+    O.set_threads(4)
+    b = O.particle_run(p, 60, xc, r)
+    O.set_threads(0)
+    assert a["rc"] == 0 and b["rc"] == 0
+    for k in ("xc", "U", "Om", "F", "T", "owner", "rho", "u", "v", "w"):
+        assert np.array_equal(a[k], b[k]), k
