@@ -122,7 +122,11 @@ typedef struct lbm_info {
     int collision;
 } lbm_info;
 
-/* Allocate device state for p.  Replaces the allocations of initial.f90:35-44. */
+/* Allocate device state for p.  Replaces the allocations of initial.f90:35-44.
+ * Isothermal FP64 and FP32 contexts launch their step kernels that read no
+ * halo with an LDS reservation (a quarter of a CU's LDS per 256-thread
+ * block) that holds them to 16 resident waves per CU, which streams faster;
+ * kernels of other work on the same device see that LDS in use meanwhile. */
 int lbm_create(const lbm_params *p, lbm_ctx **ctx);
 int lbm_destroy(lbm_ctx *ctx);
 
