@@ -55,7 +55,13 @@ def parse():
                     help="total steps after which reduced storage is compared with FP64 (default per workload)")
     ap.add_argument("--variants", default="f32,f16", help="cavity: comma list of reduced storages ('' = none)")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle on a bounded sample")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU oracle leg (0 = every host core this process may use, at most "
+                         "OMP_NUM_THREADS when that is set)")
+    ap.add_argument("--waves-per-cu", type=int, default=0,
+                    help="cavity / thermal: occupancy cap of the step kernels (0 = the library's choice)")
+    ap.add_argument("--transform", default="auto", choices=["auto", "valu", "mfma"],
+                    help="cavity: moment transforms of the FP16-stored MRT step")
     ap.add_argument("--halo-loopback", type=int, default=1,
                     help="cavity: 1 = the six faces and twelve edges exchanged every step, looped back "
                          "(written by the step kernel)")
@@ -68,7 +74,7 @@ def parse():
                     help="only run device copies of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
     defaults = {  # steps, warmup, n, accuracy steps
-        "cavity": (1000, 50, 256, 5000),
+        "cavity": (1000, 50, 256, 20000),
         "batch": (1000, 200, 0, 0),
         "particles": (1000, 20, 192, 5000),
         "thermal": (2000, 50, 128, 20000),
@@ -141,6 +147,30 @@ def line_head(args, value, ms_step, dtype, data, config):
             "vs_baseline": None, "dtype": dtype, "data": data, "config": config}
 
 
+def host_cpu():
+    """The host's CPU: model name, logical CPUs of the machine, CPUs this
+    process may run on, and the thread count a CPU leg uses by default (the
+    latter, capped by OMP_NUM_THREADS when set: a shared box's CPU share)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    threads = usable
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return {"model": model, "cpu_count": os.cpu_count(), "usable_cpus": usable, "threads": threads}
+
+
 def oracle_rate(run, cells, budget_s=15.0, max_steps=200):
     """MLUPS of an oracle run callable run(steps) -> seconds, on a sample of
     about budget_s seconds."""
@@ -155,16 +185,23 @@ def run_storage(args, storage, stream):
     from mglc_amd.lbm import Cavity, CavityConfig
 
     n = args.n
-    base = dict(Reynolds=1000.0, U0=0.1, collision=args.collision, storage=storage)
+    base = dict(Reynolds=1000.0, U0=0.1, collision=args.collision, storage=storage, waves_per_cu=args.waves_per_cu,
+                transform=args.transform if storage == "f16" else "auto")
     with Cavity(CavityConfig(n, n, n, halo_loopback=args.halo_loopback, **base), stream=stream.handle.value) as cav:
         cav.initial()
         # with the loopback every timed step is one launch of the step kernel,
         # which also writes the outgoing halo (the first, untimed step call
         # exchanged the initial state's halo once)
         ms, wall = timed(cav.step, stream, args)
+        # a step call writes the macroscopic fields on its last step only:
+        # time single-step calls (each such a step) to split the timed
+        # region into its one macro step and K - 1 plain steps
+        ms_macro, _ = timed(lambda k: [cav.step(1) for _ in range(k)], stream,
+                            argparse.Namespace(warmup=1, steps=5))
+        ms_macro /= 5
         info = cav.info()
         halo_bytes = cav.halo_buffer_bytes() if args.halo_loopback else 0
-        done = args.warmup + args.steps
+        done = args.warmup + args.steps + 6
         if args.accuracy_steps > done:
             cav.step(args.accuracy_steps - done)
         fields = cav.macro()
@@ -176,7 +213,9 @@ def run_storage(args, storage, stream):
             cav.initial()
             ms_no_exchange, _ = timed(cav.step, stream, args)
     cells = n ** 3
+    plain = (ms - ms_macro) / (args.steps - 1) if args.steps > 1 else None
     return dict(storage=storage, ms=ms, ms_kernel=ms, ms_no_exchange=ms_no_exchange, wall_s=wall,
+                ms_macro_step=ms_macro, ms_plain_step=plain,
                 mlups=cells * args.steps / (ms * 1e-3) / 1e6, bytes_per_lup=info.bytes_per_lup,
                 bytes_per_launch=info.bytes_per_lup * cells + halo_bytes, halo_bytes=halo_bytes, fields=fields,
                 steps_total=steps_total)
@@ -207,7 +246,8 @@ def cpu_baseline_cavity(args):
     workload: 256^3 cells from the reference initial state, a few steps."""
     from oracle import oracle as O
     n = args.n
-    threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+    host = host_cpu()
+    threads = args.cpu_threads if args.cpu_threads > 0 else host["threads"]
     tau = 0.1 * n / 1000.0 * 3.0 + 0.5
     coll = O.MRT if args.collision == "mrt" else O.BGK
     p = O.cavity_params(n, n, n, tau, 0.1, collision=coll)
@@ -217,25 +257,28 @@ def cpu_baseline_cavity(args):
         return O.last_loop_seconds()
 
     rate, steps = oracle_rate(run, n ** 3)
-    return {"value": rate, "unit": "MLUPS", "cores": threads, "kind": "port",
+    return {"value": rate, "unit": "MLUPS", "cores": threads, "kind": "port", "host": host,
             "sample": f"{n}^3 {args.collision.upper()} FP64 cavity, {steps} steps from the reference "
                       f"initial state, oracle/lbm_oracle.c with {threads} OpenMP threads, loop time only"}
 
 
-def cpu_reference(kind, n, steps, cells):
+def cpu_reference(kind, n, cells, budget_s=15.0, max_steps=100):
     """The reference's own subroutines (oracle/_ref, built from the reference's
-    Fortran sources, one process) timed on `steps` steps of the same n^3 grid;
-    None where that build is absent."""
+    Fortran sources, one process) timed on a sample of the same n^3 grid of
+    about budget_s seconds of loop time; None where that build is absent."""
     from oracle import oracle as O
     exe = O.REF_CAVITY if kind == "cavity" else O.REF_THERMAL
     if not os.path.exists(exe):
         return None
+    per = max(O.reference_loop_seconds(kind, n, n, n, 1), 1e-3)
+    steps = int(max(1, min(max_steps, round(budget_s / per))))
     sec = O.reference_loop_seconds(kind, n, n, n, steps)
     what = ("MPI/Lid_driven_cavity/fortran/3d/mpi_3d_blocked collision, streaming, bounceback, macro"
             if kind == "cavity" else
             "MPI/Buoyancy_driven_cavity/fortran/3d/mpi_blocked collision, streaming, bounceback, collisionT, "
             "streamingT, bouncebackT, macro, macroT")
     return {"value": cells * steps / sec / 1e6, "unit": "MLUPS", "cores": 1, "kind": "reference",
+            "host": host_cpu(), "seconds": round(sec, 2),
             "sample": f"{n}^3, {steps} steps of the reference's own {what} (amdflang -O3, one process, "
                       f"loop time only; its compile-time tau and Ra differ from this workload's, which "
                       f"changes no operation count)"}
@@ -256,6 +299,9 @@ def bench_cavity(args, stream):
                         "kernel_hbm_frac": round(r["bytes_per_launch"] / (r["ms_kernel"] / args.steps * 1e-3)
                                                  / 1e9 / HBM_PEAK_GBS, 4),
                         "ms_per_step_no_exchange": round(r["ms_no_exchange"] / args.steps, 4),
+                        "ms_macro_step": round(r["ms_macro_step"], 4),
+                        "ms_per_step_without_macro": None if r["ms_plain_step"] is None
+                        else round(r["ms_plain_step"], 4),
                         "speedup_vs_f64": round(r["mlups"] / primary["mlups"], 3),
                         "rel_l2_velocity_vs_f64": l2, "mass_drift_vs_f64": drift,
                         "compared_at_step": r["steps_total"]}
@@ -266,6 +312,10 @@ def bench_cavity(args, stream):
               "halo": "6 faces x 5 pops + 12 edges looped back every step (written by the step kernel)"
               if args.halo_loopback else "none",
               "kernel_ms_per_step": round(primary["ms_kernel"] / args.steps, 4),
+              "ms_macro_step": round(primary["ms_macro_step"], 4),
+              "ms_per_step_without_macro": None if primary["ms_plain_step"] is None
+              else round(primary["ms_plain_step"], 4),
+              "waves_per_cu": args.waves_per_cu or "library default (16)",
               "bytes_per_launch": {"lattice": int(primary["bytes_per_lup"] * args.n ** 3),
                                    "outgoing_halo": int(primary["halo_bytes"])},
               "ms_per_step_no_exchange": round(primary["ms_no_exchange"] / args.steps, 4)}
@@ -285,7 +335,7 @@ def bench_cavity(args, stream):
             v["attainable"] = attainable(st, args.n, v["kernel_hbm_frac"] * HBM_PEAK_GBS)
     if args.cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_cavity(args)
-        ref = cpu_reference("cavity", args.n, 2, args.n ** 3)
+        ref = cpu_reference("cavity", args.n, args.n ** 3)
         if ref is not None:
             line["cpu_reference"] = ref
     return line
@@ -333,7 +383,7 @@ def bench_batch(args, stream_unused):
                              "launches_per_step": len(cfgs), "speedup_batched": round(mlups / mlups_each, 3)}
     if args.cpu_baseline:
         from oracle import oracle as O
-        threads_c = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        threads_c = args.cpu_threads if args.cpu_threads > 0 else host_cpu()["threads"]
         params = [O.cavity_params(c.nx, c.ny, c.nz, c.tau(), c.U0, collision=O.MRT if args.collision == "mrt"
                                   else O.BGK) for c in cfgs]
 
@@ -346,6 +396,7 @@ def bench_batch(args, stream_unused):
 
         rate, steps = oracle_rate(run, cells, budget_s=15.0, max_steps=50)
         line["cpu_baseline"] = {"value": rate, "unit": "MLUPS", "cores": threads_c, "kind": "port",
+                                "host": host_cpu(),
                                 "sample": f"the 64 instances in FP64, {steps} steps each, one after another, "
                                           f"oracle/lbm_oracle.c with {threads_c} OpenMP threads, loop time only"}
     return line
@@ -423,7 +474,7 @@ def bench_particles(args, stream):
         p = O.particle_params(n, n, n, count, tau, rho_solid=rho_s, gravity=g,
                               collision=O.MRT if args.collision == "mrt" else O.BGK)
 
-        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        threads = args.cpu_threads if args.cpu_threads > 0 else host_cpu()["threads"]
         O.set_threads(threads)
 
         def run(steps):
@@ -435,7 +486,7 @@ def bench_particles(args, stream):
         steps = 4
         sec = run(steps) - t_init
         line["cpu_baseline"] = {"value": cells * steps / sec / 1e6, "unit": "MLUPS", "cores": threads,
-                                "kind": "port",
+                                "kind": "port", "host": host_cpu(),
                                 "sample": f"the same {n}^3 / {count}-sphere case in FP64, {steps} steps, "
                                           f"oracle/particle_oracle.c with {threads} OpenMP threads (per-node "
                                           "loops; link sums, springs and refill serial), set-up time subtracted"}
@@ -474,7 +525,7 @@ def bench_thermal(args, stream):
     if args.cpu_baseline:
         from oracle import oracle as O
         d = cfg.derived()
-        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        threads = args.cpu_threads if args.cpu_threads > 0 else host_cpu()["threads"]
         p = O.thermal_params(n, n, n, d["tau"], d["para_a"], d["g_beta"], 0.0, 0.5, -0.5, 0.0, tconst_axis=1)
         fields = (np.ones_like(T0), z, z, z, T0)
 
@@ -486,9 +537,10 @@ def bench_thermal(args, stream):
         t_init = run(0)
         rate, steps = oracle_rate(lambda s: max(run(s) - t_init, 1e-6), cells)
         line["cpu_baseline"] = {"value": rate, "unit": "MLUPS", "cores": threads, "kind": "port",
+                                "host": host_cpu(),
                                 "sample": f"the same {n}^3 case in FP64, {steps} steps, oracle/thermal_oracle.c "
                                           f"with {threads} OpenMP threads, set-up time subtracted"}
-        ref = cpu_reference("thermal", n, 6, cells)
+        ref = cpu_reference("thermal", n, cells)
         if ref is not None:
             line["cpu_reference"] = ref
     return line

@@ -98,7 +98,15 @@ typedef struct lbm_params {
                              the next step's halo, so no pack / unpack runs between steps.
                              Halo faces must be y and z faces only (x, the contiguous axis,
                              stays whole); isothermal contexts only */
-    int reserved_i[2];
+    int waves_per_cu;     /* occupancy cap of the step kernels (the fused step, the split
+                             step's interior, the thermal step), in waves per CU (a multiple
+                             of the waves of one block, at most 32), held by an LDS
+                             reservation the kernels never touch; 0 = the library's choice:
+                             16 for the isothermal FP64 / FP32 steps that read no halo, no
+                             cap otherwise */
+    int transform;        /* MRT moment transforms of FP16-stored steps: 0 = the library's
+                             choice, 1 = VALU, 2 = MFMA (matrix cores); other storages and
+                             BGK use VALU */
     double tau;           /* tauf: commondata.f90:9 tauf = 3 U0 N / Re + 0.5
                              (thermal: module.F90:19 tauf = 0.5 + Ma N sqrt(3 Pr / Ra)) */
     double u_lid;         /* U0: lid velocity along +x on the top wall k = nz (0 when thermal) */
@@ -123,10 +131,10 @@ typedef struct lbm_info {
 } lbm_info;
 
 /* Allocate device state for p.  Replaces the allocations of initial.f90:35-44.
- * Isothermal FP64 and FP32 contexts launch their step kernels that read no
- * halo with an LDS reservation (a quarter of a CU's LDS per 256-thread
- * block) that holds them to 16 resident waves per CU, which streams faster;
- * kernels of other work on the same device see that LDS in use meanwhile. */
+ * Step kernels under an occupancy cap (waves_per_cu; by default the
+ * isothermal FP64 and FP32 steps that read no halo, held to 16 waves per CU,
+ * which streams faster) launch with an LDS reservation per block; kernels of
+ * other work on the same device see that LDS in use meanwhile. */
 int lbm_create(const lbm_params *p, lbm_ctx **ctx);
 int lbm_destroy(lbm_ctx *ctx);
 
@@ -181,9 +189,10 @@ int lbm_check(lbm_ctx *ctx, double *error_u);
  * errorT = sum |T - Tp| / sum |T|; then up, vp, wp, Tp = u, v, w, T. */
 int lbm_check_thermal(lbm_ctx *ctx, double *error_u, double *error_t);
 
-/* Nusselt numbers across the held walls of a thermal context (FP64):
- * nu_vol  = <u_a T> N_a / kappa + 1, the convective form of RaNu.F90:11-24
- *           with u_a the velocity along the held axis;
+/* Nusselt numbers of a thermal context (FP64):
+ * nu_vol  = <w T> nz / kappa + 1, calNuRe's volume Nusselt number
+ *           (RaNu.F90:13-23): w is the velocity along gravity (z), whichever
+ *           walls are held;
  * nu_wall = mean over the hot wall of 2 (t_hot - T_first_layer) N_a / (t_hot - t_cold),
  *           the conductive flux at the half-link wall.  kappa = (paraA + 6) / (42 sqrt 3). */
 int lbm_nusselt(lbm_ctx *ctx, double *nu_vol, double *nu_wall);

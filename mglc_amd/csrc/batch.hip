@@ -360,14 +360,17 @@ int lbm_batch_create(const lbm_params* params, int count, lbm_batch** out) {
             return lbm_fail(LBM_ERR_ARG, "lbm_batch_create: instance %d grid %d x %d x %d", m, p.nx, p.ny, p.nz);
         if (!(p.tau > 0.5) || !std::isfinite(p.tau))
             return lbm_fail(LBM_ERR_TAU, "lbm_batch_create: instance %d tau = %g", m, p.tau);
-        if (p.storage != p0.storage || p.collision != p0.collision || p.meq12_rho != p0.meq12_rho)
-            return lbm_fail(LBM_ERR_ARG, "lbm_batch_create: instance %d: storage, collision and meq12_rho must "
-                                         "match instance 0", m);
+        if (p.storage != p0.storage || p.collision != p0.collision || p.meq12_rho != p0.meq12_rho ||
+            p.f16_scale_log2 != p0.f16_scale_log2 || p.rho0 != p0.rho0)
+            return lbm_fail(LBM_ERR_ARG, "lbm_batch_create: instance %d: storage, collision, meq12_rho, "
+                                         "f16_scale_log2 and rho0 must match instance 0", m);
         if (p.thermal || p.halo_faces || p.halo_loopback)
             return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_batch_create: batched instances are isothermal cavities");
     }
     if (p0.storage < LBM_STORE_F64 || p0.storage > LBM_STORE_F16 || (p0.collision != LBM_MRT && p0.collision != LBM_BGK))
         return lbm_fail(LBM_ERR_ARG, "lbm_batch_create: storage %d / collision %d", p0.storage, p0.collision);
+    if (p0.f16_scale_log2 < 0 || p0.f16_scale_log2 > 20)
+        return lbm_fail(LBM_ERR_ARG, "lbm_batch_create: f16_scale_log2 %d out of [0, 20]", p0.f16_scale_log2);
 
     lbm_batch* b = new (std::nothrow) lbm_batch();
     if (!b) return lbm_fail(LBM_ERR_NOMEM, "lbm_batch_create: host allocation failed");

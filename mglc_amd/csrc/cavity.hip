@@ -740,6 +740,10 @@ int lbm_create(const lbm_params* p, lbm_ctx** out) {
     }
     if (p->f16_scale_log2 < 0 || p->f16_scale_log2 > 20)
         return lbm_fail(LBM_ERR_ARG, "lbm_create: f16_scale_log2 %d out of [0, 20]", p->f16_scale_log2);
+    if (p->transform < 0 || p->transform > 2)
+        return lbm_fail(LBM_ERR_ARG, "lbm_create: transform %d (0 library's choice, 1 VALU, 2 MFMA)", p->transform);
+    if (p->transform == 2)
+        return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_create: no MFMA moment transform in this build");
     // default launch shape: one block per 256-cell x-row run (measured best at
     // 256^3: whole rows per workgroup share their partial edge cache lines)
     const int bx_def = p->nx >= 256 ? 256 : (p->nx + 63) / 64 * 64;
@@ -749,6 +753,9 @@ int lbm_create(const lbm_params* p, lbm_ctx** out) {
         return lbm_fail(LBM_ERR_ARG, "lbm_create: block %d x %d (x must be a multiple of 64, <= 256 threads)",
                     bx, by);
 
+    if (p->waves_per_cu < 0 || p->waves_per_cu > 32 || p->waves_per_cu % ((bx * by + 63) / 64) != 0)
+        return lbm_fail(LBM_ERR_ARG, "lbm_create: waves_per_cu %d is not a multiple of the %d waves of a block "
+                                     "in [0, 32]", p->waves_per_cu, (bx * by + 63) / 64);
     if (p->halo_faces < 0 || p->halo_faces > 63)
         return lbm_fail(LBM_ERR_ARG, "lbm_create: halo_faces 0x%x is not a 6-bit face mask", p->halo_faces);
     if (p->halo_overlap != 0 && p->halo_overlap != 1)
@@ -819,18 +826,33 @@ int lbm_create(const lbm_params* p, lbm_ctx** out) {
     // 2.6-3.6 % in FP32; FP16, bound by VALU issue, loses 25 % and keeps every
     // wave, and so do the kernels that read a halo (4 % slower capped: more
     // registers and work per wave).  The cap is an LDS reservation per block
-    // (the CU's LDS over the blocks of 16 waves) that the kernels never touch.
-    if (!p->thermal && p->storage != LBM_STORE_F16) {
+    // (the CU's LDS over the blocks of the allowed waves) that the kernels
+    // never touch.  An explicit waves_per_cu applies to the fused step (also
+    // when it reads a halo), the split step's interior and the thermal step.
+    {
         const int waves_per_block = (int)((c->block.x * c->block.y + 63) / 64);
-        int lds_cu = 0, lds_block = 0;
-        if (16 % waves_per_block == 0 &&
-            hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device) ==
-                hipSuccess &&
-            hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess) {
-            const int reserve = lds_cu / (16 / waves_per_block);
-            if (reserve > 0 && reserve <= lds_block) c->step_lds = (unsigned)reserve;
+        const int policy = (!p->thermal && p->storage != LBM_STORE_F16) ? 16 : 0;
+        const int want = p->waves_per_cu ? p->waves_per_cu : policy;
+        if (want > 0 && want % waves_per_block == 0) {
+            int lds_cu = 0, lds_block = 0;
+            unsigned reserve = 0;
+            if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device) ==
+                    hipSuccess &&
+                hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) ==
+                    hipSuccess) {
+                const int r = lds_cu / (want / waves_per_block);
+                if (r > 0 && r <= lds_block) reserve = (unsigned)r;
+            }
+            (void)hipGetLastError();
+            if (!reserve && p->waves_per_cu) {
+                delete c;
+                return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_create: the device cannot hold %d waves per CU by an "
+                                                     "LDS reservation (%d bytes of LDS per CU, %d per block)",
+                                p->waves_per_cu, lds_cu, lds_block);
+            }
+            c->step_lds = reserve;
+            c->step_lds_halo = p->waves_per_cu ? reserve : 0u;
         }
-        (void)hipGetLastError();
     }
     const size_t lat = (size_t)c->ps * Q * c->elem;
     hipError_t e = hipSuccess;
@@ -1000,7 +1022,8 @@ int lbm_step(lbm_ctx* c, int nsteps) {
         int rc = fresh_loopback_halo(c);
         if (rc) return rc;
     }
-    const unsigned lds = c->p.halo_faces ? 0u : c->step_lds;  // the halo-reading kernel keeps every wave
+    const unsigned lds = c->p.halo_faces ? c->step_lds_halo : c->step_lds;  // by default the halo-reading
+                                                                            // kernel keeps every wave
     for (int s = 0; s < nsteps; ++s) {
         a.src = c->buf[c->cur];
         a.dst = c->buf[1 - c->cur];

@@ -46,6 +46,7 @@ struct lbm_ctx {
     dim3 block{64, 4, 1};
     unsigned step_lds = 0;       // LDS bytes reserved per block of the step kernels: caps the
                                  // blocks resident per CU (0: no cap); see lbm_create
+    unsigned step_lds_halo = 0;  // the same for the kernels that read a halo (and the thermal step)
     float scale = 1.0f, inv_scale = 1.0f;
     long long device_bytes = 0;
     long long hoff[lbm::NREGION] = {}, hcount[lbm::NREGION] = {}, htotal = 0;

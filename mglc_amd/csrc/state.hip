@@ -25,17 +25,29 @@
 
 namespace {
 
-constexpr char STATE_MAGIC[8] = {'M', 'G', 'L', 'C', 'S', 'T', 'A', '1'};
+constexpr char STATE_MAGIC[8] = {'M', 'G', 'L', 'C', 'S', 'T', 'A', '2'};
 
+// What a blob's populations mean: besides the grid and storage, FP16 blobs
+// hold (f - w) * 2^scale_log2, and the halo flags decide which halo the
+// next step reads and whether it writes one.  A context must match all of
+// them to resume.
 struct StateHeader {
     char magic[8];
     int32_t header_bytes;
     int32_t nx, ny, nz;
     int32_t storage, collision, thermal, halo_faces;
+    int32_t scale_log2;     // the FP16 scale exponent in effect (0 for FP64 / FP32)
+    int32_t halo_loopback, halo_overlap;
     int32_t reserved;
     int64_t ps, htotal, itc;
     int64_t total_bytes;
 };
+
+// The FP16 scale exponent in effect for c (f16_scale_log2 = 0 selects 12).
+int32_t scale_log2_of(const lbm_ctx* c) {
+    if (c->p.storage != LBM_STORE_F16) return 0;
+    return c->p.f16_scale_log2 ? c->p.f16_scale_log2 : 12;
+}
 
 struct Section {
     void* dev;
@@ -105,6 +117,9 @@ int lbm_save_state(lbm_ctx* c, void* blob, size_t bytes) {
     h.collision = c->p.collision;
     h.thermal = c->p.thermal;
     h.halo_faces = c->p.halo_faces;
+    h.scale_log2 = scale_log2_of(c);
+    h.halo_loopback = c->p.halo_loopback;
+    h.halo_overlap = c->p.halo_overlap;
     h.ps = c->ps;
     h.htotal = c->htotal;
     h.itc = c->itc;
@@ -138,6 +153,14 @@ int lbm_load_state(lbm_ctx* c, const void* blob, size_t bytes) {
                         "context, this one is %dx%dx%d storage %d collision %d thermal %d halo 0x%x",
                         h.nx, h.ny, h.nz, h.storage, h.collision, h.thermal, h.halo_faces, c->p.nx, c->p.ny,
                         c->p.nz, c->p.storage, c->p.collision, c->p.thermal, c->p.halo_faces);
+    if (h.scale_log2 != scale_log2_of(c))
+        return lbm_fail(LBM_ERR_ARG,
+                        "lbm_load_state: FP16 state scaled by 2^%d, this context scales by 2^%d "
+                        "(f16_scale_log2)", h.scale_log2, scale_log2_of(c));
+    if (h.halo_loopback != c->p.halo_loopback || h.halo_overlap != c->p.halo_overlap)
+        return lbm_fail(LBM_ERR_ARG,
+                        "lbm_load_state: state of a context with halo_loopback %d halo_overlap %d, this one "
+                        "has %d %d", h.halo_loopback, h.halo_overlap, c->p.halo_loopback, c->p.halo_overlap);
     const size_t need = blob_bytes(c);
     if (h.total_bytes != (int64_t)need || bytes < need)
         return lbm_fail(LBM_ERR_ARG, "lbm_load_state: state of %lld bytes, %zu given, %zu expected",

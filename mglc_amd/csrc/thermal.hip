@@ -238,9 +238,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_check_thermal(const double* __r
     block_store<4>(e, partial);
 }
 
-// Nusselt sums: s0 = sum ua T over all cells, s1 = sum 2 (Thot - T) over the
-// hot-wall layer (first layer along the held axis).
-__global__ __launch_bounds__(RED_THREADS) void k_nusselt(const double* __restrict__ ua,
+// Nusselt sums: s0 = sum w T over all cells (RaNu.F90:13-21), s1 = sum
+// 2 (Thot - T) over the hot-wall layer (first layer along the held axis).
+__global__ __launch_bounds__(RED_THREADS) void k_nusselt(const double* __restrict__ w,
                                                          const double* __restrict__ T, int nx, int ny, int nz,
                                                          int taxis, double t_hot, double* __restrict__ partial) {
     double e[2] = {0.0, 0.0};
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_nusselt(const double* __restric
     for (long long c = blockIdx.x * (long long)RED_THREADS + threadIdx.x; c < n;
          c += (long long)gridDim.x * RED_THREADS) {
         const double t = T[c];
-        e[0] += ua[c] * t;
+        e[0] += w[c] * t;
         const long long i = c % nx, j = (c / nx) % ny, k = c / ((long long)nx * ny);
         const long long q = taxis == 0 ? i : (taxis == 1 ? j : k);
         if (q == 0) e[1] += (t_hot - t) * 2.0;
@@ -360,7 +360,7 @@ int thermal_step(lbm_ctx* c, int nsteps) {
         const bool macro = s == nsteps - 1;
         void (*kern)(const ThermalArgs) = rot ? (macro ? k_thermal_step<true, true> : k_thermal_step<true, false>)
                                               : (macro ? k_thermal_step<false, true> : k_thermal_step<false, false>);
-        hipLaunchKernelGGL(kern, grid, c->block, 0, c->stream, a);
+        hipLaunchKernelGGL(kern, grid, c->block, c->step_lds_halo, c->stream, a);
         c->cur = 1 - c->cur;
     }
     return lbm_check_launch();
@@ -403,8 +403,8 @@ int flow_reynolds(lbm_ctx* c, double* re) {
 int thermal_nusselt(lbm_ctx* c, double* nu_vol, double* nu_wall) {
     const int ax = c->p.thermal_axis;
     if (ax < 0 || ax > 2) return lbm_fail(LBM_ERR_ARG, "lbm_nusselt: no held walls (thermal_axis = %d)", ax);
-    const double* ua = c->macro + (1 + ax) * c->n;
-    hipLaunchKernelGGL(k_nusselt, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, ua, c->temp, c->p.nx, c->p.ny,
+    const double* w = c->macro + 3 * c->n;  // gravity acts along z
+    hipLaunchKernelGGL(k_nusselt, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, w, c->temp, c->p.nx, c->p.ny,
                        c->p.nz, ax, c->p.t_hot, c->red);
     int rc = lbm_check_launch();
     if (rc) return rc;
@@ -414,7 +414,8 @@ int thermal_nusselt(lbm_ctx* c, double* nu_vol, double* nu_wall) {
     const double kappa = (c->p.para_a + 6.0) / (42.0 * std::sqrt(3.0));  // inverse of module.F90:27
     const int na = ax == 0 ? c->p.nx : (ax == 1 ? c->p.ny : c->p.nz);
     const long long nwall = c->n / na;
-    *nu_vol = s[0] / (double)c->n * (double)na / kappa + 1.0;
+    // calNuRe (RaNu.F90:21): <w T> nz / diffusivity + 1, whichever walls are held
+    *nu_vol = s[0] / (double)c->n * (double)c->p.nz / kappa + 1.0;
     *nu_wall = s[1] / (double)nwall * (double)na / (c->p.t_hot - c->p.t_cold);
     return LBM_OK;
 }

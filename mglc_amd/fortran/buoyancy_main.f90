@@ -75,7 +75,8 @@ program buoyancy_main
     p%halo_loopback = 0
     p%thermal_axis = heated_axis
     p%halo_overlap = 0
-    p%reserved_i = 0
+    p%waves_per_cu = 0
+    p%transform = 0
     p%tau = tauf
     p%u_lid = 0.0d0
     p%rho0 = 1.0d0

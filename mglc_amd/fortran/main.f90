@@ -59,7 +59,8 @@ program main
     p%halo_loopback = merge(1, 0, halo_loopback)
     p%thermal_axis = -1
     p%halo_overlap = 0
-    p%reserved_i = 0
+    p%waves_per_cu = 0
+    p%transform = 0
     p%tau = tauf
     p%u_lid = U0
     p%rho0 = rho0

@@ -34,7 +34,8 @@ module mglc_lbm_capi
         integer(c_int) :: halo_loopback
         integer(c_int) :: thermal_axis
         integer(c_int) :: halo_overlap
-        integer(c_int) :: reserved_i(2)
+        integer(c_int) :: waves_per_cu
+        integer(c_int) :: transform
         real(c_double) :: tau
         real(c_double) :: u_lid
         real(c_double) :: rho0

@@ -23,6 +23,7 @@ LBM_MRT, LBM_BGK = 0, 1
 STORE_F64, STORE_F32, STORE_F16 = 0, 1, 2
 STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16}
 COLLISION_NAMES = {"mrt": LBM_MRT, "bgk": LBM_BGK}
+TRANSFORM_NAMES = {"auto": 0, "valu": 1, "mfma": 2}
 
 LBM_OK, LBM_ERR_NAN = 0, 3
 
@@ -50,7 +51,7 @@ class LbmParams(ctypes.Structure):
         ("f16_scale_log2", ctypes.c_int), ("thermal", ctypes.c_int),
         ("block_x", ctypes.c_int), ("block_y", ctypes.c_int),
         ("halo_faces", ctypes.c_int), ("halo_loopback", ctypes.c_int), ("thermal_axis", ctypes.c_int),
-        ("halo_overlap", ctypes.c_int), ("reserved_i", ctypes.c_int * 2),
+        ("halo_overlap", ctypes.c_int), ("waves_per_cu", ctypes.c_int), ("transform", ctypes.c_int),
         ("tau", ctypes.c_double), ("u_lid", ctypes.c_double), ("rho0", ctypes.c_double),
         ("para_a", ctypes.c_double), ("g_beta", ctypes.c_double), ("omega_rot", ctypes.c_double),
         ("t_hot", ctypes.c_double), ("t_cold", ctypes.c_double), ("t_ref", ctypes.c_double),
@@ -192,6 +193,8 @@ class CavityConfig:
     halo_faces: int = 0        # bit f: face f reads a halo (FACE_XM..FACE_ZP) instead of a wall
     halo_loopback: int = 0     # 1: exchange looped back onto this subdomain before every step
     halo_overlap: int = 0      # 1: boundary shell + interior per step, exchange overlapped with the interior
+    waves_per_cu: int = 0      # occupancy cap of the step kernels (0 = the library's choice)
+    transform: str = "auto"    # FP16 MRT moment transforms: "auto", "valu" or "mfma"
 
     def tau(self) -> float:
         if self.tauf is not None:
@@ -207,6 +210,8 @@ class CavityConfig:
         p.f16_scale_log2 = self.f16_scale_log2
         p.block_x, p.block_y = self.block_x, self.block_y
         p.halo_faces, p.halo_loopback, p.halo_overlap = self.halo_faces, self.halo_loopback, self.halo_overlap
+        p.waves_per_cu = self.waves_per_cu
+        p.transform = TRANSFORM_NAMES[self.transform]
         p.tau, p.u_lid, p.rho0 = self.tau(), self.U0, self.rho0
         return p
 
@@ -449,7 +454,8 @@ class Cavity:
         return eu.value, et.value
 
     def nusselt(self):
-        """(nu_vol, nu_wall) across the held walls."""
+        """(nu_vol, nu_wall): calNuRe's <w T> nz / kappa + 1 (RaNu.F90:13-23) and
+        the conductive flux through the hot wall."""
         a, b = ctypes.c_double(), ctypes.c_double()
         self._call("lbm_nusselt", self._ctx, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value

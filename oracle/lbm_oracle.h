@@ -97,7 +97,7 @@ void oracle_thermal_initial_T(const oracle_thermal_params *p, double *T);
 void oracle_guo_source_moments(double u, double v, double w, double Fx, double Fy, double Fz,
                                double *S);
 void oracle_d3q7_basis(double N[7][7], double Ninv[7][7]);
-void oracle_thermal_nusselt(const oracle_thermal_params *p, const double *ua, const double *T,
+void oracle_thermal_nusselt(const oracle_thermal_params *p, const double *w, const double *T,
                             double kappa, double *nu_vol, double *nu_wall);
 
 /* ---- moving spheres (D3Q19 generalisation of case4/mpi) ------------------- */

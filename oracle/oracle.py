@@ -214,8 +214,10 @@ def d3q7_basis():
     return N, Ninv
 
 
-def thermal_nusselt(p: ThermalParams, ua, T, kappa):
-    a = np.ascontiguousarray(ua, dtype=np.float64)
+def thermal_nusselt(p: ThermalParams, w, T, kappa):
+    """(nu_vol, nu_wall): calNuRe's <w T> nz / kappa + 1 (RaNu.F90:13-23) and
+    the conductive flux through the hot wall."""
+    a = np.ascontiguousarray(w, dtype=np.float64)
     t = np.ascontiguousarray(T, dtype=np.float64)
     nv, nw = ctypes.c_double(), ctypes.c_double()
     _thermal_lib().oracle_thermal_nusselt(ctypes.byref(p), _dp(a), _dp(t), kappa, ctypes.byref(nv), ctypes.byref(nw))

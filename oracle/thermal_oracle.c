@@ -327,22 +327,23 @@ int oracle_thermal_run(const oracle_thermal_params *p, int nsteps, int check_eve
     return nchk;
 }
 
-/* Nusselt numbers across the held walls (axis a = tconst_axis, hot at the
- * low side), FP64:
- *   nu_vol  = <u_a T> N_a / kappa + 1 -- the convective form of RaNu.F90:11-24
- *             and of the 2-D side-heat template (2d/mpi_blocked/output.F90:142-172),
- *             with the velocity component along the heated axis;
- *   nu_wall = mean over the hot wall of (Thot - T(first layer)) / (1/2) * N_a /
- *             (Thot - Tcold): the conductive flux through the wall at the half-link.
- * `ua` is the velocity component along the heated axis. */
-void oracle_thermal_nusselt(const oracle_thermal_params *p, const double *ua, const double *T, double kappa,
+/* Nusselt numbers of the buoyancy cavity, FP64:
+ *   nu_vol  = <w T> nz / kappa + 1 -- calNuRe's volume Nusselt number
+ *             (RaNu.F90:13-23): the velocity along gravity (z) and the z
+ *             extent, whichever walls are held;
+ *   nu_wall = mean over the hot wall (axis a = tconst_axis, hot at the low
+ *             side) of (Thot - T(first layer)) / (1/2) * N_a / (Thot - Tcold):
+ *             the conductive flux through the wall at the half-link.
+ * `w` is the z velocity; the volume sum runs over the cells in storage
+ * order (x fastest), as RaNu.F90's k, j, i loops do. */
+void oracle_thermal_nusselt(const oracle_thermal_params *p, const double *w, const double *T, double kappa,
                             double *nu_vol, double *nu_wall) {
     size_t n = (size_t)p->nx * p->ny * p->nz;
     const int ax = p->tconst_axis;
     const int na = ax == 0 ? p->nx : (ax == 1 ? p->ny : p->nz);
     double s = 0.0;
-    for (size_t c = 0; c < n; ++c) s += ua[c] * T[c];
-    *nu_vol = s / (double)n * (double)na / kappa + 1.0;
+    for (size_t c = 0; c < n; ++c) s += w[c] * T[c];
+    *nu_vol = s / (double)n * (double)p->nz / kappa + 1.0;
     double sw = 0.0;
     size_t nwall = 0;
     for (int k = 0; k < p->nz; ++k)

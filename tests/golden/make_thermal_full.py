@@ -36,7 +36,7 @@ def main():
     rho, u, v, w, T, errs = O.thermal_run(p, steps, fields=(np.ones_like(T0), z, z, z, T0), check_every=2000,
                                           nthreads=threads)
     print(f"oracle {N}^3 x {steps} steps: {time.time() - t0:.0f} s")
-    nu = O.thermal_nusselt(p, v, T, d["diffusivity"])
+    nu = O.thermal_nusselt(p, w, T, d["diffusivity"])
     h = N // 2
     s = slice(STRIDE // 2, None, STRIDE)
     name = os.path.join(HERE, f"thermal_full_{N}_s{steps}.npz")

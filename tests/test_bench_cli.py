@@ -22,3 +22,18 @@ def test_roofline_fields():
     assert abs(r["achieved"] - 5100.0) < 1e-6
     assert abs(r["frac"] - 5100.0 / 8000.0) < 1e-4
     assert r["traffic"] == 5.2e9
+
+
+def test_host_cpu_reports_model_and_thread_count(monkeypatch):
+    """The CPU legs state what they ran on: the CPU model, the machine's CPU
+    count, the CPUs this process may use, and a default thread count capped
+    by OMP_NUM_THREADS (a shared box's CPU share)."""
+    sys.path.insert(0, ROOT)
+    import bench
+    monkeypatch.setenv("OMP_NUM_THREADS", "3")
+    h = bench.host_cpu()
+    assert isinstance(h["model"], str) and h["model"]
+    assert h["cpu_count"] >= h["usable_cpus"] >= 1
+    assert h["threads"] == min(3, h["usable_cpus"])
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    assert bench.host_cpu()["threads"] == h["usable_cpus"]

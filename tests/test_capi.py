@@ -138,3 +138,21 @@ def test_create_validates_the_split_step(hip_lib):
     p.halo_overlap = 1
     assert hip_lib.lbm_create(ctypes.byref(p), ctypes.byref(ctx)) == 7
     assert hip_lib.lbm_step_begin(None) == 1 and hip_lib.lbm_step_end(None) == 1
+
+
+def test_batch_create_checks_the_fp16_scale_and_density(hip_lib):
+    """f16_scale_log2 within [0, 20] (2^k must stay a finite float scale), and
+    the same scale and rho0 in every instance: the batch stores all instances
+    with instance 0's.  Refused before any device allocation."""
+    from mglc_amd import lbm
+    b = ctypes.c_void_p()
+    bad = lbm.CavityConfig(16, 16, 16, storage="f16", f16_scale_log2=130).params()
+    assert hip_lib.lbm_batch_create((lbm.LbmParams * 1)(bad), 1, ctypes.byref(b)) == 1
+    assert b"f16_scale_log2" in hip_lib.lbm_last_error() and not b.value
+    p0 = lbm.CavityConfig(16, 16, 16, storage="f16", f16_scale_log2=12).params()
+    p1 = lbm.CavityConfig(16, 16, 16, storage="f16", f16_scale_log2=10).params()
+    assert hip_lib.lbm_batch_create((lbm.LbmParams * 2)(p0, p1), 2, ctypes.byref(b)) == 1
+    assert b"f16_scale_log2" in hip_lib.lbm_last_error() and not b.value
+    p2 = lbm.CavityConfig(16, 16, 16, storage="f16", f16_scale_log2=12, rho0=1.5).params()
+    assert hip_lib.lbm_batch_create((lbm.LbmParams * 2)(p0, p2), 2, ctypes.byref(b)) == 1
+    assert b"rho0" in hip_lib.lbm_last_error() and not b.value

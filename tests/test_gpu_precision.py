@@ -1,0 +1,49 @@
+"""The precision form of one GPU's subdomain, at full size: the lid-driven
+cavity, D3Q19 MRT, 256^3, Re = 1000, U0 = 0.1 (tau = 0.5768), 20000 steps,
+with the six-face + twelve-edge halo exchange looped back every step.
+Populations stored as FP64, as FP32 and as FP16 shifted deviations (f - w)
+2^12 (FP32 arithmetic, FP64 reductions); the reduced storages are held to
+the FP64 GPU path on the relative L2 error of (u, v, w) and on the drift of
+the total mass:
+
+    FP16: L2 <= 1e-3, mass drift <= 1e-6
+    FP32: L2 <= 1e-4, mass drift <= 1e-7
+"""
+import numpy as np
+import pytest
+
+from mglc_amd.lbm import Cavity, CavityConfig
+
+pytestmark = pytest.mark.gpu
+
+N, STEPS, CHUNK = 256, 20000, 2000
+BOUNDS = {"f32": (1e-4, 1e-7), "f16": (1e-3, 1e-6)}
+
+
+def run(storage):
+    cfg = CavityConfig(N, N, N, Reynolds=1000.0, U0=0.1, collision="mrt", storage=storage, halo_loopback=1)
+    with Cavity(cfg) as c:
+        c.initial()
+        for s in range(0, STEPS, CHUNK):
+            print(f"[gpu] precision {storage} {N}^3 step {s} -> {s + CHUNK}", flush=True)
+            c.step(CHUNK)
+        assert c.info().steps_done == STEPS
+        rho, u, v, w = c.macro()
+    return float(rho.sum()), np.stack([u, v, w])
+
+
+@pytest.fixture(scope="module")
+def fp64():
+    return run("f64")
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_reduced_storage_at_full_size(fp64, storage):
+    mass64, vel64 = fp64
+    mass, vel = run(storage)
+    l2 = float(np.linalg.norm((vel - vel64).ravel()) / np.linalg.norm(vel64.ravel()))
+    drift = abs(mass - mass64) / mass64
+    print(f"{storage}: relative L2 velocity error {l2:.3e}, mass drift {drift:.3e} after {STEPS} steps", flush=True)
+    l2_bound, drift_bound = BOUNDS[storage]
+    assert l2 <= l2_bound
+    assert drift <= drift_bound

@@ -74,6 +74,27 @@ def test_state_is_refused_by_another_context():
         assert e.value.status == LBM_ERR_ARG
 
 
+def test_state_keeps_the_fp16_scale_and_halo_flags():
+    """An FP16 state holds (f - w) 2^k: a context with another k would misread
+    every population by 2^(k1 - k2), so it refuses the state; so does a context
+    whose halo loopback or split step differs from the saved one's."""
+    with Cavity(CavityConfig(24, 24, 24, storage="f16", f16_scale_log2=12)) as c:
+        c.initial()
+        c.step(3)
+        blob = c.save_state()
+    with Cavity(CavityConfig(24, 24, 24, storage="f16", f16_scale_log2=10)) as d:
+        with pytest.raises(LbmError) as e:
+            d.load_state(blob)
+        assert e.value.status == LBM_ERR_ARG and "2^12" in str(e.value)
+    # k = 0 selects the default 12: the same scale, accepted
+    with Cavity(CavityConfig(24, 24, 24, storage="f16", f16_scale_log2=0)) as d:
+        d.load_state(blob)
+    with Cavity(CavityConfig(24, 24, 24, storage="f16", f16_scale_log2=12, halo_loopback=1)) as d:
+        with pytest.raises(LbmError) as e:
+            d.load_state(blob)
+        assert e.value.status == LBM_ERR_ARG and "halo_loopback" in str(e.value)
+
+
 def test_contexts_with_spheres_are_not_checkpointed():
     with Cavity(CavityConfig(24, 24, 24, U0=0.0, tauf=0.8, meq12_rho=1)) as c:
         c.particles_init(np.array([[12.0, 12.0, 12.0]]), np.array([3.0]))

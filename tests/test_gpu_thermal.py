@@ -89,7 +89,7 @@ def test_fp64_thermal_matches_oracle_with_noise(oracle_lib, shape, steps):
     for g, r, name in zip(got[1:4], ref[1:4], "uvw"):
         assert np.max(np.abs(g - r)) <= 1e-12 * U, name
     np.testing.assert_allclose([e1, e2], ref[5], rtol=1e-9)
-    ref_nu = O.thermal_nusselt(p, ref[2], ref[4], d["diffusivity"])
+    ref_nu = O.thermal_nusselt(p, ref[3], ref[4], d["diffusivity"])
     np.testing.assert_allclose(nu, ref_nu, rtol=1e-11)
 
 

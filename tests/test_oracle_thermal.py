@@ -94,7 +94,7 @@ def test_conduction_steady_state_is_linear_with_unit_nusselt(oracle_lib):
     y = np.arange(ny) + 0.5
     expect = 0.5 - y / ny
     assert np.abs(T - expect[None, :, None]).max() < 1e-10
-    nu_vol, nu_wall = O.thermal_nusselt(p, v, T, kappa)
+    nu_vol, nu_wall = O.thermal_nusselt(p, w, T, kappa)
     assert nu_vol == pytest.approx(1.0, abs=1e-12)
     assert nu_wall == pytest.approx(1.0, abs=1e-9)
 
