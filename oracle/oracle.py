@@ -255,8 +255,15 @@ def particle_params(nx, ny, nz, np_, tau, rho_solid=1.5, gravity=1e-5, rho0=1.0,
                           stiff_wall, threshold_particle, stiff_particle)
 
 
-def _particle_lib():
-    L = lib()
+def particle_library(path: str) -> ctypes.CDLL:
+    """Another build of the oracle library (a copy compiled elsewhere, e.g.
+    with a formula altered on purpose to show that a test catches it), set
+    up for particle_run(..., L=)."""
+    return _particle_lib(ctypes.CDLL(path))
+
+
+def _particle_lib(L=None):
+    L = lib() if L is None else L
     if not getattr(L, "_particle_ready", False):
         dp = ctypes.POINTER(ctypes.c_double)
         ip = ctypes.POINTER(ctypes.c_int)
@@ -296,18 +303,19 @@ def calq(xc, r, pid, x, a):
     return rc, q.value, x0, closed
 
 
-def set_threads(n: int) -> None:
+def set_threads(n: int, L=None) -> None:
     """Threads of the oracle's OpenMP loops that take no thread count of
     their own (the particle oracles' per-node loops)."""
-    L = lib()
+    L = lib() if L is None else L
     L.oracle_set_threads.argtypes = [ctypes.c_int]
     L.oracle_set_threads.restype = None
     L.oracle_set_threads(int(n))
 
 
-def particle_run(p: ParticleParams, nsteps: int, xc, r, U=None, Om=None):
+def particle_run(p: ParticleParams, nsteps: int, xc, r, U=None, Om=None, L=None):
     """Spheres at xc (np x 3, 1-based node coordinates) with radii r, fluid at
-    rest; returns a dict of the state after nsteps (rc != 0 on a stop)."""
+    rest; returns a dict of the state after nsteps (rc != 0 on a stop).  L:
+    a library from particle_library (default: this directory's build)."""
     n = p.np
     xc = np.ascontiguousarray(xc, dtype=np.float64).reshape(n, 3).copy()
     r = np.ascontiguousarray(r, dtype=np.float64)
@@ -318,7 +326,7 @@ def particle_run(p: ParticleParams, nsteps: int, xc, r, U=None, Om=None):
     owner = np.zeros(shape, dtype=np.int32)
     rho, u, v, w = (np.zeros(shape) for _ in range(4))
     ra = ctypes.c_double()
-    rc = _particle_lib().oracle_particle_run(ctypes.byref(p), nsteps, _dp(xc), _dp(U), _dp(Om), _dp(r), _dp(F), _dp(T),
+    rc = _particle_lib(L).oracle_particle_run(ctypes.byref(p), nsteps, _dp(xc), _dp(U), _dp(Om), _dp(r), _dp(F), _dp(T),
                                              _ip(owner), _dp(rho), _dp(u), _dp(v), _dp(w), ctypes.byref(ra))
     return dict(rc=rc, xc=xc, U=U, Om=Om, F=F, T=T, owner=owner, rho=rho, u=u, v=v, w=w, rho_avg=ra.value)
 

@@ -13,11 +13,15 @@
  * the 2-D restatement of the reference's sequential program
  * (particle2d_oracle.c) also uses and which that restatement checks bit for
  * bit against states the reference program itself wrote.  The 3-D assembly
- * around them is pinned by closed forms (the particle map against
- * brute-force enumeration, the bisection wall distance against the
- * reference's own closed form calQ2, interpolated bounce-back reducing to
- * halfway bounce-back at q = 1/2, rest-state invariance); the 3-D
- * generalisation choices below have no reference output to compare with.
+ * around them has no reference output to compare with; it is pinned by
+ * closed forms and physics (tests/test_oracle_particle_pins.py,
+ * tests/test_oracle_particles.py): the particle map against brute-force
+ * enumeration, the bisection wall distance against the reference's own closed
+ * form calQ2, rest-state invariance, the solid sphere's m and I through the
+ * kinematic update, the buoyancy volume, the six wall springs, the refill's
+ * direction, extrapolation and momentum on every uncovered node, Stokes
+ * spin-down and settling in the closed box; altering any of those formulas
+ * makes a named check fail.
  *
  * Step (main.F90:35-69): collision on fluid nodes -> masked pull streaming ->
  * box-wall bounce-back -> interpolated bounce-back on particle links with the
