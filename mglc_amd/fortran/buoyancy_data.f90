@@ -111,8 +111,15 @@ contains
         endif
         NuVolAvg(dimensionlessTime) = nu_vol
         ReVolAvg(dimensionlessTime) = re_vol
-        NuVolAvg_mean(dimensionlessTime) = sum(NuVolAvg(1:dimensionlessTime)) / dble(dimensionlessTime)
-        ReVolAvg_mean(dimensionlessTime) = sum(ReVolAvg(1:dimensionlessTime)) / dble(dimensionlessTime)
+        ! running means summed in time order, as RaNu.F90:29-33 and 48-52 do
+        nf = 0.0d0
+        rf = 0.0d0
+        do q = 1, dimensionlessTime
+            nf = nf + NuVolAvg(q)
+            rf = rf + ReVolAvg(q)
+        enddo
+        NuVolAvg_mean(dimensionlessTime) = nf / dble(dimensionlessTime)
+        ReVolAvg_mean(dimensionlessTime) = rf / dble(dimensionlessTime)
 
         open(newunit=unit, file=trim(output_dir)//'/Nu_Bulk.dat', status='unknown', position='append')
         write(unit, *) itc / timeUnit, nu_vol
@@ -121,29 +128,51 @@ contains
         write(unit, *) itc / timeUnit, re_vol
         close(unit)
 
+        ! the notes of statistically.txt are the reference's lines, word for
+        ! word (RaNu.F90:59-132), so the file reads and diffs the same
         if (statisticallyStationaryState == 1) then
             if (mod(dimensionlessTime - statisticallyStationaryTime, 200) == 0) then
                 halfTime = (statisticallyStationaryTime + dimensionlessTime) / 2
-                nf = sum(NuVolAvg(statisticallyStationaryTime:halfTime)) / dble(halfTime - statisticallyStationaryTime + 1)
-                rf = sum(ReVolAvg(statisticallyStationaryTime:halfTime)) / dble(halfTime - statisticallyStationaryTime + 1)
-                nl = sum(NuVolAvg(halfTime:dimensionlessTime)) / dble(dimensionlessTime - halfTime + 1)
-                rl = sum(ReVolAvg(halfTime:dimensionlessTime)) / dble(dimensionlessTime - halfTime + 1)
+                nf = 0.0d0
+                rf = 0.0d0
+                do q = statisticallyStationaryTime, halfTime
+                    nf = nf + NuVolAvg(q)
+                    rf = rf + ReVolAvg(q)
+                enddo
+                nf = nf / dble(halfTime - statisticallyStationaryTime + 1)
+                rf = rf / dble(halfTime - statisticallyStationaryTime + 1)
+                nl = 0.0d0
+                rl = 0.0d0
+                do q = halfTime, dimensionlessTime
+                    nl = nl + NuVolAvg(q)
+                    rl = rl + ReVolAvg(q)
+                enddo
+                nl = nl / dble(dimensionlessTime - halfTime + 1)
+                rl = rl / dble(dimensionlessTime - halfTime + 1)
                 dn = dabs((nf - nl) / nl)
                 dr = dabs((rf - rl) / rl)
-                open(newunit=unit, file=trim(output_dir)//'/statistically.txt', status='unknown', position='append')
                 if (dn <= 0.01d0 .and. dr <= 0.01d0) then
                     if (flowReversal == 0) then
                         statisticallyConvergeState = 1
                     else if (dimensionlessTime >= statisticallyStationaryTime + flowReversalTime) then
                         statisticallyConvergeState = 1
                     endif
-                    write(unit, *) 'Statistically Converge Reached at Time =', dimensionlessTime
+                    open(newunit=unit, file=trim(output_dir)//'/statistically.txt', status='unknown', &
+                         position='append')
+                    write(unit, *) '|-------------------------------------------------------------------------|'
+                    write(unit, *) '|---------Statistically Converge Reached at Time =', dimensionlessTime, &
+                                   '----------|'
+                    write(unit, *) '|-------------------------------------------------------------------------|'
+                    write(unit, *) '   '
+                    close(unit)
                 endif
+                open(newunit=unit, file=trim(output_dir)//'/statistically.txt', status='unknown', position='append')
                 write(unit, *) 'statisticallyStationaryTime=', statisticallyStationaryTime
                 write(unit, *) 'halfTime =', halfTime
                 write(unit, *) 'dimensionlessTime =', dimensionlessTime
                 write(unit, *) 'NuVolAvg diff. (first and last half):', dn * 100.0d0, '%'
                 write(unit, *) 'ReVolAvg diff. (first and last half):', dr * 100.0d0, '%'
+                write(unit, *) '               '
                 close(unit)
             endif
         endif
@@ -154,15 +183,24 @@ contains
                           NuVolAvg_mean(dimensionlessTime))
                 dr = dabs((ReVolAvg_mean(dimensionlessTime) - ReVolAvg_mean(dimensionlessTime - 100)) / &
                           ReVolAvg_mean(dimensionlessTime))
-                open(newunit=unit, file=trim(output_dir)//'/statistically.txt', status='unknown', position='append')
                 if (dn <= 0.01d0 .and. dr <= 0.01d0) then
                     statisticallyStationaryState = 1
                     statisticallyStationaryTime = dimensionlessTime
-                    write(unit, *) 'Statistically Stationary Reached at Time =', statisticallyStationaryTime
+                    open(newunit=unit, file=trim(output_dir)//'/statistically.txt', status='unknown', &
+                         position='append')
+                    write(unit, *) '|-------------------------------------------------------------------------|'
+                    write(unit, *) '|-----------Statistically Stationary Reached at Time =', &
+                                   statisticallyStationaryTime, '------|'
+                    write(unit, *) '|-------------------------------------------------------------------------|'
+                    write(unit, *) '   '
+                    write(unit, *) '   '
+                    close(unit)
                 endif
+                open(newunit=unit, file=trim(output_dir)//'/statistically.txt', status='unknown', position='append')
                 write(unit, *) 'dimensionlessTime =', dimensionlessTime
                 write(unit, *) 'NuVolAvg diff. (100 timeunit interval):', dn * 100.0d0, '%'
                 write(unit, *) 'ReVolAvg diff. (100 timeunit interval):', dr * 100.0d0, '%'
+                write(unit, *) '               '
                 close(unit)
             endif
         endif
