@@ -262,26 +262,32 @@ def cpu_baseline_cavity(args):
                       f"initial state, oracle/lbm_oracle.c with {threads} OpenMP threads, loop time only"}
 
 
-def cpu_reference(kind, n, cells, budget_s=15.0, max_steps=100):
-    """The reference's own subroutines (oracle/_ref, built from the reference's
-    Fortran sources, one process) timed on a sample of the same n^3 grid of
-    about budget_s seconds of loop time; None where that build is absent."""
+def cpu_reference(kind, n, cells, threads, budget_s=15.0, max_steps=100):
+    """The reference's own subroutines (oracle/_ref, built from the
+    reference's Fortran sources) on the host cores: `threads` processes at
+    once, each stepping one z slab of the n^3 grid (the subdomains of the
+    reference's MPI decomposition, without its halo exchange, so an upper
+    bound on that run's rate), timed on a sample of about budget_s seconds of
+    loop time; None where that build is absent."""
     from oracle import oracle as O
     exe = O.REF_CAVITY if kind == "cavity" else O.REF_THERMAL
     if not os.path.exists(exe):
         return None
-    per = max(O.reference_loop_seconds(kind, n, n, n, 1), 1e-3)
+    procs = max(1, min(threads, n // 2))
+    nzs = [n // procs + (1 if p < n % procs else 0) for p in range(procs)]
+    per = max(max(O.reference_loop_seconds_slabs(kind, n, n, nzs, 1)), 1e-3)
     steps = int(max(1, min(max_steps, round(budget_s / per))))
-    sec = O.reference_loop_seconds(kind, n, n, n, steps)
+    sec = max(O.reference_loop_seconds_slabs(kind, n, n, nzs, steps))
     what = ("MPI/Lid_driven_cavity/fortran/3d/mpi_3d_blocked collision, streaming, bounceback, macro"
             if kind == "cavity" else
             "MPI/Buoyancy_driven_cavity/fortran/3d/mpi_blocked collision, streaming, bounceback, collisionT, "
             "streamingT, bouncebackT, macro, macroT")
-    return {"value": cells * steps / sec / 1e6, "unit": "MLUPS", "cores": 1, "kind": "reference",
+    return {"value": cells * steps / sec / 1e6, "unit": "MLUPS", "cores": procs, "kind": "reference",
             "host": host_cpu(), "seconds": round(sec, 2),
-            "sample": f"{n}^3, {steps} steps of the reference's own {what} (amdflang -O3, one process, "
-                      f"loop time only; its compile-time tau and Ra differ from this workload's, which "
-                      f"changes no operation count)"}
+            "sample": f"{n}^3 as {procs} z slabs of {min(nzs)}-{max(nzs)} layers, one process per slab at once, "
+                      f"{steps} steps of the reference's own {what} (amdflang -O3; loop time of the slowest "
+                      f"process; no halo exchange between slabs; its compile-time tau and Ra differ from this "
+                      f"workload's, which changes no operation count)"}
 
 
 def bench_cavity(args, stream):
@@ -335,7 +341,8 @@ def bench_cavity(args, stream):
             v["attainable"] = attainable(st, args.n, v["kernel_hbm_frac"] * HBM_PEAK_GBS)
     if args.cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_cavity(args)
-        ref = cpu_reference("cavity", args.n, args.n ** 3)
+        threads = args.cpu_threads if args.cpu_threads > 0 else host_cpu()["threads"]
+        ref = cpu_reference("cavity", args.n, args.n ** 3, threads)
         if ref is not None:
             line["cpu_reference"] = ref
     return line
@@ -540,7 +547,7 @@ def bench_thermal(args, stream):
                                 "host": host_cpu(),
                                 "sample": f"the same {n}^3 case in FP64, {steps} steps, oracle/thermal_oracle.c "
                                           f"with {threads} OpenMP threads, set-up time subtracted"}
-        ref = cpu_reference("thermal", n, cells)
+        ref = cpu_reference("thermal", n, cells, threads)
         if ref is not None:
             line["cpu_reference"] = ref
     return line

@@ -137,6 +137,30 @@ def reference_loop_seconds(kind, nx, ny, nz, nsteps):
     raise RuntimeError(f"{exe} printed no ref_loop_seconds line")
 
 
+def reference_loop_seconds_slabs(kind, nx, ny, nzs, nsteps):
+    """The reference's own step loop (as reference_loop_seconds) run as
+    len(nzs) processes at once, process p on an nx x ny x nzs[p] slab: the
+    subdomains of a z decomposition of the reference's MPI run, each a whole
+    walled cavity, without the halo exchange between them.  Returns each
+    process's loop seconds."""
+    import tempfile
+    exe = REF_CAVITY if kind == "cavity" else REF_THERMAL
+    with tempfile.TemporaryDirectory() as d:
+        procs = [subprocess.Popen([exe, str(nx), str(ny), str(nz), str(nsteps), os.path.join(d, f"fields{p}.bin")],
+                                  stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+                 for p, nz in enumerate(nzs)]
+        outs = [pr.communicate() for pr in procs]
+    secs = []
+    for pr, (out, err) in zip(procs, outs):
+        if pr.returncode != 0:
+            raise RuntimeError(f"{exe} failed ({pr.returncode}): {err[-500:]}")
+        vals = [float(l.split()[1]) for l in out.splitlines() if l.startswith("ref_loop_seconds")]
+        if not vals:
+            raise RuntimeError(f"{exe} printed no ref_loop_seconds line")
+        secs.append(vals[0])
+    return secs
+
+
 # ---- buoyancy-driven cavity (D3Q19 + Guo forcing, D3Q7 temperature) ----------
 REF_THERMAL = os.path.join(HERE, "_ref", "ref_thermal")
 
