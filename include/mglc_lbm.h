@@ -210,6 +210,20 @@ int lbm_reynolds(lbm_ctx *ctx, double *re);
  * subdomains and takes errorU = sqrt(e1) / sqrt(e2). */
 int lbm_check_sums(lbm_ctx *ctx, double *e1, double *e2);
 
+/* The sums of lbm_check_sums over the caller's own device arrays, allocating
+ * nothing.  No counterpart in the reference: for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  u, v, w, up, vp: distinct device arrays of n doubles;
+ * e1 = sum (u-up)^2 + (v-vp)^2 and e2 = sum u^2 + v^2 + w^2 are written to
+ * sums_dev[0], sums_dev[1] (device memory), then up = u, vp = v; the sums equal
+ * lbm_check_sums's bit for bit on the same fields.  Queued on hip_stream
+ * (NULL = null stream) without waiting.  The per-block partial sums live in a
+ * fixed library array of 2 x 1024 doubles (16 KiB) per device, whatever n;
+ * every call on a device shares it, so calls that overlap on one GPU (other
+ * streams, other threads) race.  n in [1, 2^40], else LBM_ERR_ARG. */
+int lbm_check_sums_noalloc(const double *u, const double *v, const double *w, double *up, double *vp,
+                           long long n, double *sums_dev, void *hip_stream);
+
 /* Packed halo buffer: all 18 regions concatenated in region order, elements of
  * the storage type (8, 4 or 2 bytes).  Faces: [slot 0..4][plane], slots the 5
  * populations in ascending order, plane index j + ny*k (x faces), i + nx*k

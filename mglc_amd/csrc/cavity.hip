@@ -24,6 +24,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "collide.hpp"
@@ -682,6 +683,44 @@ int lbm_launch_cavity_step(const StepArgs& a, int storage, int collision, bool m
     return lbm_check_launch();
 }
 
+// ---- scratch of the entry point that allocates nothing -------------------------------
+// lbm_check_sums_noalloc keeps its per-block partial sums (the scratch of
+// k_check_partial / k_reduce_pairs) in this fixed array instead of a context's
+// buffer: 2 x RED_BLOCKS doubles (16 KiB), one copy per device, held until the
+// process ends.  In host code `scratch` names a host-side stand-in, not a
+// device address: host code never passes it to a kernel or a copy, but looks
+// up each device's own address once (noalloc_scratch) and passes that.
+static __device__ __align__(256) double scratch[2 * RED_BLOCKS];
+
+namespace {
+constexpr long long NOALLOC_MAX_CELLS = 1LL << 40;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[NOALLOC_MAX_DEVICES];  // device address of `scratch` per device, once looked up
+
+// The current device's address of `scratch`, looked up (hipGetSymbolAddress,
+// with that device current) on the first call on each device.
+int noalloc_scratch(double** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return lbm_fail(LBM_ERR_HIP, "lbm_check_sums_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_check_sums_noalloc: device %d beyond the first %d", dev,
+                        NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return lbm_fail(LBM_ERR_HIP, "lbm_check_sums_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                            hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return LBM_OK;
+}
+}  // namespace
+
 int lbm_fail(int code, const char* fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
@@ -1063,6 +1102,21 @@ int lbm_check_sums(lbm_ctx* c, double* e1, double* e2) {
     *e1 = tot[0];
     *e2 = tot[1];
     return LBM_OK;
+}
+
+int lbm_check_sums_noalloc(const double* u, const double* v, const double* w, double* up, double* vp,
+                           long long n, double* sums_dev, void* hip_stream) {
+    if (!u || !v || !w || !up || !vp || !sums_dev)
+        return lbm_fail(LBM_ERR_ARG, "lbm_check_sums_noalloc: null argument");
+    if (n < 1 || n > NOALLOC_MAX_CELLS)
+        return lbm_fail(LBM_ERR_ARG, "lbm_check_sums_noalloc: n = %lld cells outside [1, 2^40]", n);
+    double* partial = nullptr;
+    int rc = noalloc_scratch(&partial);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(hip_stream);
+    hipLaunchKernelGGL(k_check_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, u, v, w, up, vp, n, partial);
+    hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(RED_THREADS), 0, s, partial, RED_BLOCKS, sums_dev);
+    return lbm_check_launch();
 }
 
 int lbm_check(lbm_ctx* c, double* error_u) {

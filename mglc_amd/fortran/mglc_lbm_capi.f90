@@ -171,6 +171,15 @@ module mglc_lbm_capi
             real(c_double), intent(out) :: e1, e2
         end function lbm_check_sums
 
+        ! the same sums over the caller's device arrays (c_ptr device addresses),
+        ! allocating nothing; the sums land in device memory at sums_dev
+        integer(c_int) function lbm_check_sums_noalloc(u, v, w, up, vp, n, sums_dev, hip_stream) &
+                bind(C, name='lbm_check_sums_noalloc')
+            import :: c_int, c_ptr, c_long_long
+            type(c_ptr), value :: u, v, w, up, vp, sums_dev, hip_stream
+            integer(c_long_long), value :: n
+        end function lbm_check_sums_noalloc
+
         ! the subdomain exchange (ex_sendrecv.f90:12-123): device buffers
         integer(c_int) function lbm_halo_layout(ctx, region, offset, count) bind(C, name='lbm_halo_layout')
             import :: c_int, c_ptr, c_long_long

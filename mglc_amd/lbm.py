@@ -37,7 +37,7 @@ EXPORTED_SYMBOLS = (
     "lbm_batch_autotune", "lbm_batch_set_launch", "lbm_batch_synchronize", "lbm_batch_download_macro",
     "lbm_batch_get_info", "lbm_particles_init", "lbm_particles_get", "lbm_particles_map",
     "lbm_state_bytes", "lbm_save_state", "lbm_load_state", "lbm_reynolds", "lbm_bandwidth_probe",
-    "lbm_step_begin", "lbm_step_end",
+    "lbm_step_begin", "lbm_step_end", "lbm_check_sums_noalloc",
 )
 
 FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = range(6)
@@ -121,6 +121,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lbm_synchronize.argtypes = [vp]
     lib.lbm_get_info.argtypes = [vp, ctypes.POINTER(LbmInfo)]
     lib.lbm_check_sums.argtypes = [vp, dp, dp]
+    lib.lbm_check_sums_noalloc.argtypes = [vp, vp, vp, vp, vp, ctypes.c_longlong, vp, vp]
     lib.lbm_halo_layout.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     lib.lbm_halo_buffer_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
     lib.lbm_halo_pack.argtypes = [vp, vp]

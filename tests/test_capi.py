@@ -156,3 +156,14 @@ def test_batch_create_checks_the_fp16_scale_and_density(hip_lib):
     p2 = lbm.CavityConfig(16, 16, 16, storage="f16", f16_scale_log2=12, rho0=1.5).params()
     assert hip_lib.lbm_batch_create((lbm.LbmParams * 2)(p0, p2), 2, ctypes.byref(b)) == 1
     assert b"rho0" in hip_lib.lbm_last_error() and not b.value
+
+
+def test_check_sums_noalloc_rejects_bad_arguments(hip_lib):
+    """lbm_check_sums_noalloc validates its pointers and cell count before any
+    device call."""
+    p = ctypes.c_void_p(4096)
+    assert hip_lib.lbm_check_sums_noalloc(None, p, p, p, p, 8, p, None) == 1
+    assert hip_lib.lbm_check_sums_noalloc(p, p, p, p, p, 8, None, None) == 1
+    assert hip_lib.lbm_check_sums_noalloc(p, p, p, p, p, 0, p, None) == 1
+    assert hip_lib.lbm_check_sums_noalloc(p, p, p, p, p, (1 << 40) + 1, p, None) == 1
+    assert b"outside [1, 2^40]" in hip_lib.lbm_last_error()
