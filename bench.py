@@ -53,7 +53,8 @@ def parse():
     ap.add_argument("--collision", default="mrt", choices=["mrt", "bgk"])
     ap.add_argument("--accuracy-steps", type=int, default=None,
                     help="total steps after which reduced storage is compared with FP64 (default per workload)")
-    ap.add_argument("--variants", default="f32,f16", help="cavity: comma list of reduced storages ('' = none)")
+    ap.add_argument("--variants", default="f32,f16",
+                    help="cavity / thermal: comma list of reduced storages ('' = none)")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle on a bounded sample")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads of the CPU oracle leg (0 = every host core this process may use, at most "
@@ -70,6 +71,9 @@ def parse():
                          "interior)")
     ap.add_argument("--attainable", type=int, default=1,
                     help="cavity / batch: also time the step's traffic with no arithmetic (lbm_bandwidth_probe)")
+    ap.add_argument("--device-warmup", type=int, default=1,
+                    help="cavity / thermal: before the W warmup steps, step a throwaway context of the same configuration "
+                         "for ~150 ms of device time (steady clocks; the measured state is untouched)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -103,6 +107,29 @@ def timed(step, stream, args):
     e0.close()
     e1.close()
     return ms, wall
+
+
+def device_warmup(make_ctx, stream, ms=150.0, chunk=20):
+    """Bring the device to its steady clocks before a measurement: step a
+    throwaway context of the same configuration (its state is discarded, the
+    measured context's is untouched) for about `ms` of device time.  A GPU
+    that idled while the host worked (downloads, the previous variant's
+    comparison) starts slow; the VALU-bound FP16 step is the most
+    clock-sensitive."""
+    from mglc_amd.hip import Event
+    done = 0.0
+    with make_ctx() as w:
+        w.initial()
+        while done < ms:
+            e0, e1 = Event(), Event()
+            e0.record(stream)
+            w.step(chunk)
+            e1.record(stream)
+            e1.synchronize()
+            done += e0.elapsed_ms(e1)
+            e0.close()
+            e1.close()
+    return round(done, 1)
 
 
 def traffic_from_profiles(workload_key):
@@ -187,8 +214,12 @@ def run_storage(args, storage, stream):
     n = args.n
     base = dict(Reynolds=1000.0, U0=0.1, collision=args.collision, storage=storage, waves_per_cu=args.waves_per_cu,
                 transform=args.transform if storage == "f16" else "auto")
-    with Cavity(CavityConfig(n, n, n, halo_loopback=args.halo_loopback, **base), stream=stream.handle.value) as cav:
+    def make():
+        return Cavity(CavityConfig(n, n, n, halo_loopback=args.halo_loopback, **base), stream=stream.handle.value)
+
+    with make() as cav:
         cav.initial()
+        warm_ms = device_warmup(make, stream) if args.device_warmup else 0.0
         # with the loopback every timed step is one launch of the step kernel,
         # which also writes the outgoing halo (the first, untimed step call
         # exchanged the initial state's halo once)
@@ -214,10 +245,13 @@ def run_storage(args, storage, stream):
             ms_no_exchange, _ = timed(cav.step, stream, args)
     cells = n ** 3
     plain = (ms - ms_macro) / (args.steps - 1) if args.steps > 1 else None
-    return dict(storage=storage, ms=ms, ms_kernel=ms, ms_no_exchange=ms_no_exchange, wall_s=wall,
+    return dict(storage=storage, ms=ms, ms_kernel=ms, ms_no_exchange=ms_no_exchange, wall_s=wall, warm_ms=warm_ms,
                 ms_macro_step=ms_macro, ms_plain_step=plain,
                 mlups=cells * args.steps / (ms * 1e-3) / 1e6, bytes_per_lup=info.bytes_per_lup,
-                bytes_per_launch=info.bytes_per_lup * cells + halo_bytes, halo_bytes=halo_bytes, fields=fields,
+                # a step call writes rho, u, v, w (FP64) on its last step: the K
+                # timed launches write them once
+                bytes_per_launch=info.bytes_per_lup * cells + halo_bytes + 32.0 * cells / args.steps,
+                halo_bytes=halo_bytes, fields=fields,
                 steps_total=steps_total)
 
 
@@ -310,7 +344,7 @@ def bench_cavity(args, stream):
                         else round(r["ms_plain_step"], 4),
                         "speedup_vs_f64": round(r["mlups"] / primary["mlups"], 3),
                         "rel_l2_velocity_vs_f64": l2, "mass_drift_vs_f64": drift,
-                        "compared_at_step": r["steps_total"]}
+                        "compared_at_step": r["steps_total"], "device_warmup_ms": r["warm_ms"]}
     m0 = float(primary["fields"][0].size)
     workload = f"lid-driven cavity D3Q19 {args.n}^3 Re=1000 U0=0.1 {args.collision.upper()}"
     config = {"workload": workload, "cells": args.n ** 3, "collision": args.collision,
@@ -323,8 +357,10 @@ def bench_cavity(args, stream):
               else round(primary["ms_plain_step"], 4),
               "waves_per_cu": args.waves_per_cu or "library default (16)",
               "bytes_per_launch": {"lattice": int(primary["bytes_per_lup"] * args.n ** 3),
+                                   "macro_fields_once_per_K_launches": 32 * args.n ** 3,
                                    "outgoing_halo": int(primary["halo_bytes"])},
-              "ms_per_step_no_exchange": round(primary["ms_no_exchange"] / args.steps, 4)}
+              "ms_per_step_no_exchange": round(primary["ms_no_exchange"] / args.steps, 4),
+              "device_warmup_ms": primary["warm_ms"]}
     line = line_head(args, primary["mlups"], primary["ms"] / args.steps, "f64",
                      "synthetic (reference initial state: rho=1, u=0, lid layer u=U0)", config)
     line["roofline"] = roofline(primary["bytes_per_launch"], primary["ms_kernel"] / args.steps,
@@ -505,30 +541,64 @@ def bench_thermal(args, stream):
     from mglc_amd.lbm import Cavity, ThermalConfig, buoyancy_initial_temperature
 
     n = args.n
-    cfg = ThermalConfig(n, n, n, Rayleigh=1e5, Prandtl=0.71, Thot=0.5, Tcold=-0.5, thermal_axis=1)
     T0 = buoyancy_initial_temperature(n, n, n, seed=2024)
     z = np.zeros_like(T0)
-    with Cavity(cfg, stream=stream.handle.value) as cav:
-        cav.load_fields(np.ones_like(T0), z, z, z, T0)
-        ms, _ = timed(cav.step, stream, args)
-        done = args.warmup + args.steps
-        if args.accuracy_steps > done:
-            cav.step(args.accuracy_steps - done)
-        nu_vol, nu_wall = cav.nusselt()
-        info = cav.info()
+
+    def make(storage):
+        return ThermalConfig(n, n, n, Rayleigh=1e5, Prandtl=0.71, Thot=0.5, Tcold=-0.5, thermal_axis=1,
+                             storage=storage, waves_per_cu=args.waves_per_cu)
+
+    def run(storage):
+        with Cavity(make(storage), stream=stream.handle.value) as cav:
+            cav.load_fields(np.ones_like(T0), z, z, z, T0)
+            warm = 0.0
+            if args.device_warmup:
+                warm = device_warmup(lambda: Cavity(make(storage), stream=stream.handle.value), stream)
+            ms, _ = timed(cav.step, stream, args)
+            done = args.warmup + args.steps
+            if args.accuracy_steps > done:
+                cav.step(args.accuracy_steps - done)
+            out = dict(ms=ms, nu=cav.nusselt(), info=cav.info(), fields=cav.macro(), T=cav.temperature(),
+                       warm=warm)
+        return out
+
+    cfg = make("f64")
+    primary = run("f64")
     cells = n ** 3
+    ms = primary["ms"]
     mlups = cells * args.steps / (ms * 1e-3) / 1e6
     # k_thermal_step: 19 + 7 FP64 populations read and written, Fz read and written
-    bpl = 2 * (19 + 7) * 8 + 2 * 8
+    bpl = primary["info"].bytes_per_lup
+    nu_vol, nu_wall = primary["nu"]
     config = {"workload": f"buoyancy-driven cavity D3Q19+D3Q7 {n}^3 Ra=1e5 Pr=0.71 walls +-0.5, MRT, FP64",
-              "cells": cells, "storage": "f64", "kernel": "k_thermal_step", "bytes_per_lup": bpl}
+              "cells": cells, "storage": "f64", "kernel": "k_thermal_step", "bytes_per_lup": bpl,
+              "device_warmup_ms": primary["warm"]}
     line = line_head(args, mlups, ms / args.steps, "f64",
                      "synthetic (rho=1, u=0, T=0 plus seeded noise +-1e-4, held wall layers)", config)
     line["roofline"] = roofline(bpl * cells, ms / args.steps, traffic_from_profiles(f"thermal{n} f64"))
     if args.attainable:
         line["roofline"]["attainable"] = attainable("f64", n, line["roofline"]["achieved"], thermal=True)
-    line["nusselt"] = {"at_step": max(args.warmup + args.steps, args.accuracy_steps), "nu_volume": nu_vol,
-                       "nu_wall": nu_wall}
+    steps_total = max(args.warmup + args.steps, args.accuracy_steps)
+    line["nusselt"] = {"at_step": steps_total, "nu_volume": nu_vol, "nu_wall": nu_wall}
+    variants = {}
+    vel64 = np.stack(primary["fields"][1:])
+    for st in [s for s in args.variants.split(",") if s]:
+        r = run(st)
+        vel = np.stack(r["fields"][1:])
+        m64 = float(primary["fields"][0].sum())
+        b = r["info"].bytes_per_lup
+        variants[st] = {
+            "mlups": round(cells * args.steps / (r["ms"] * 1e-3) / 1e6, 1),
+            "ms_per_step": round(r["ms"] / args.steps, 4), "bytes_per_lup": b,
+            "kernel_hbm_frac": round(b * cells / (r["ms"] / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "speedup_vs_f64": round(ms / r["ms"], 3),
+            "rel_l2_velocity_vs_f64": float(np.linalg.norm((vel - vel64).ravel()) / np.linalg.norm(vel64.ravel())),
+            "rel_l2_T_vs_f64": float(np.linalg.norm((r["T"] - primary["T"]).ravel()) /
+                                     np.linalg.norm(primary["T"].ravel())),
+            "mass_drift_vs_f64": abs(float(r["fields"][0].sum()) - m64) / m64,
+            "nu_volume": r["nu"][0], "nu_wall": r["nu"][1], "compared_at_step": steps_total,
+            "device_warmup_ms": r["warm"]}
+    line["variants"] = variants
     if args.cpu_baseline:
         from oracle import oracle as O
         d = cfg.derived()

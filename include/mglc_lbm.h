@@ -105,8 +105,9 @@ typedef struct lbm_params {
                              16 for the isothermal FP64 / FP32 steps that read no halo, no
                              cap otherwise */
     int transform;        /* MRT moment transforms of FP16-stored steps: 0 = the library's
-                             choice, 1 = VALU, 2 = MFMA (matrix cores); other storages and
-                             BGK use VALU */
+                             choice (VALU), 1 = VALU, 2 = the forward transform on the
+                             matrix cores (MFMA; measured slower than VALU on MI355X, kept
+                             as a selectable form); other storages and BGK use VALU */
     double tau;           /* tauf: commondata.f90:9 tauf = 3 U0 N / Re + 0.5
                              (thermal: module.F90:19 tauf = 0.5 + Ma N sqrt(3 Pr / Ra)) */
     double u_lid;         /* U0: lid velocity along +x on the top wall k = nz (0 when thermal) */
@@ -325,6 +326,7 @@ typedef struct lbm_batch_info {
     double bytes_per_lup;       /* 2*19*sizeof(store) */
     int threads, blocks;        /* persistent launch shape in use */
     int graph_steps;            /* steps per captured graph */
+    int cells_per_thread;       /* cells each thread updates per step (1, 2 or 4) */
     void *stream;               /* the batch's own hipStream_t */
 } lbm_batch_info;
 
@@ -342,9 +344,14 @@ int lbm_batch_step(lbm_batch *batch, int nsteps);
  * against. */
 int lbm_batch_step_each(lbm_batch *batch, int nsteps);
 /* Time trial_steps steps at each candidate launch shape (64/128/256 threads,
- * 4..64 waves per CU), keep the fastest, and restart from the initial state. */
+ * 4..64 waves per CU, 1/2/4 cells per thread), keep the fastest, and restart
+ * from the initial state.  The cells per thread chosen are in lbm_batch_info. */
 int lbm_batch_autotune(lbm_batch *batch, int trial_steps, int *threads, int *blocks);
 int lbm_batch_set_launch(lbm_batch *batch, int threads, int blocks);
+/* Each wave takes cells_per_thread consecutive 64-cell tiles per turn (1, 2
+ * or 4): a thread updates that many cells per step.  Results do not depend on
+ * it. */
+int lbm_batch_set_cells_per_thread(lbm_batch *batch, int cells_per_thread);
 int lbm_batch_synchronize(lbm_batch *batch);
 int lbm_batch_download_macro(lbm_batch *batch, int instance, double *rho, double *u, double *v,
                              double *w);

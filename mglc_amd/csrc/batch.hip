@@ -68,7 +68,10 @@ struct InstView {
     unsigned seed;
 };
 
-template <class S, int COLL, bool MACRO>
+// CPT: cells per thread -- each wave takes CPT consecutive 64-cell tiles per
+// turn (one cell per lane from each), so a thread updates CPT cells per step
+// in a run of CPT * 64 consecutive cells.
+template <class S, int COLL, bool MACRO, int CPT>
 __global__ __launch_bounds__(256) void k_batch_step(const BatchArgs b) {
     using T = typename S::T;
     const int lane = threadIdx.x & 63;
@@ -76,7 +79,8 @@ __global__ __launch_bounds__(256) void k_batch_step(const BatchArgs b) {
     const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
     const unsigned seed = *b.step_ctr + b.step_add;
     int m = 0;  // instance of the current tile; tiles only move forward, so m does too
-    for (int t = wave; t < b.total_tiles; t += nwaves) {
+    for (int t0 = wave * CPT; t0 < b.total_tiles; t0 += nwaves * CPT)
+    for (int t = t0; t < t0 + CPT && t < b.total_tiles; ++t) {
         while (m + 1 < b.ninst && b.inst[m + 1].tile_begin <= t) ++m;
         const BatchInst& in = b.inst[m];
         // the tile's first cell, once per wave (uniform, 32-bit), then each
@@ -204,11 +208,17 @@ __global__ void k_counter_add(unsigned* ctr, unsigned n) { *ctr = *ctr + n; }
 using BatchStepFn = void (*)(const BatchArgs);
 using BatchInitFn = void (*)(const BatchArgs);
 
+// cells per thread the kernels are built for (the autotuner's choices)
+constexpr int CPT_CHOICES[3] = {1, 2, 4};
+
 template <class S>
 struct BatchTable {
-    static constexpr BatchStepFn step[2][2] = {
-        {k_batch_step<S, COLL_MRT, false>, k_batch_step<S, COLL_MRT, true>},
-        {k_batch_step<S, COLL_BGK, false>, k_batch_step<S, COLL_BGK, true>}};
+    // step[collision][macro][cells per thread: 1, 2, 4]
+    static constexpr BatchStepFn step[2][2][3] = {
+        {{k_batch_step<S, COLL_MRT, false, 1>, k_batch_step<S, COLL_MRT, false, 2>, k_batch_step<S, COLL_MRT, false, 4>},
+         {k_batch_step<S, COLL_MRT, true, 1>, k_batch_step<S, COLL_MRT, true, 2>, k_batch_step<S, COLL_MRT, true, 4>}},
+        {{k_batch_step<S, COLL_BGK, false, 1>, k_batch_step<S, COLL_BGK, false, 2>, k_batch_step<S, COLL_BGK, false, 4>},
+         {k_batch_step<S, COLL_BGK, true, 1>, k_batch_step<S, COLL_BGK, true, 2>, k_batch_step<S, COLL_BGK, true, 4>}}};
     static constexpr BatchInitFn init[2] = {k_batch_init<S, COLL_MRT>, k_batch_init<S, COLL_BGK>};
 };
 
@@ -233,10 +243,11 @@ struct lbm_batch {
     bool initialised = false;
     int threads = 256;             // launch shape of the persistent kernel
     int blocks = 0;
+    int cpt = 1;                   // cells per thread (1, 2 or 4)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_cur = -1;            // buffer parity the graph was captured for
-    int graph_threads = 0, graph_blocks = 0;
+    int graph_threads = 0, graph_blocks = 0, graph_cpt = 0;
     std::vector<StepArgs> single; // per-instance arguments of the single-domain kernel (baseline)
     long long device_bytes = 0;
 };
@@ -262,10 +273,11 @@ BatchArgs batch_args(const lbm_batch* b) {
 
 BatchStepFn batch_step_fn(const lbm_batch* b, bool macro) {
     const int coll = b->p0.collision == LBM_BGK ? 1 : 0;
+    const int ci = b->cpt == 4 ? 2 : (b->cpt == 2 ? 1 : 0);
     switch (b->p0.storage) {
-    case LBM_STORE_F32: return BatchTable<StoreF32>::step[coll][macro];
-    case LBM_STORE_F16: return BatchTable<StoreF16>::step[coll][macro];
-    default: return BatchTable<StoreF64>::step[coll][macro];
+    case LBM_STORE_F32: return BatchTable<StoreF32>::step[coll][macro][ci];
+    case LBM_STORE_F16: return BatchTable<StoreF16>::step[coll][macro][ci];
+    default: return BatchTable<StoreF64>::step[coll][macro][ci];
     }
 }
 
@@ -322,6 +334,7 @@ int capture_graph(lbm_batch* b, int cur) {
     b->graph_cur = cur;
     b->graph_threads = b->threads;
     b->graph_blocks = b->blocks;
+    b->graph_cpt = b->cpt;
     return LBM_OK;
 }
 
@@ -496,7 +509,7 @@ int lbm_batch_step(lbm_batch* b, int nsteps) {
     int remaining = nsteps;
     while (remaining >= GRAPH_STEPS) {
         if (!b->graph_exec || b->graph_cur != b->cur || b->graph_threads != b->threads ||
-            b->graph_blocks != b->blocks) {
+            b->graph_blocks != b->blocks || b->graph_cpt != b->cpt) {
             int rc = capture_graph(b, b->cur);
             if (rc) return rc;
         }
@@ -547,11 +560,13 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, int* best_threads, int* be
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     float best = 1e30f;
-    int bt = b->threads, bb = b->blocks;
+    int bt = b->threads, bb = b->blocks, bc = b->cpt;
     const int threads_c[3] = {64, 128, 256};
     const int waves_per_cu_c[5] = {4, 8, 16, 32, 64};
+    for (int ci = 0; ci < 3; ++ci)
     for (int ti = 0; ti < 3; ++ti)
         for (int wi = 0; wi < 5; ++wi) {
+            b->cpt = CPT_CHOICES[ci];
             b->threads = threads_c[ti];
             b->blocks = std::max(1, cus * waves_per_cu_c[wi] / (threads_c[ti] / 64));
             launch_batch_step(b, b->cur, false, 0);  // warm-up
@@ -570,12 +585,14 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, int* best_threads, int* be
                 best = ms;
                 bt = b->threads;
                 bb = b->blocks;
+                bc = b->cpt;
             }
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->threads = bt;
     b->blocks = bb;
+    b->cpt = bc;
     if (best_threads) *best_threads = bt;
     if (best_blocks) *best_blocks = bb;
     rc = lbm_check_launch();
@@ -590,6 +607,14 @@ int lbm_batch_set_launch(lbm_batch* b, int threads, int blocks) {
         return lbm_fail(LBM_ERR_ARG, "lbm_batch_set_launch: %d threads x %d blocks", threads, blocks);
     b->threads = threads;
     b->blocks = blocks;
+    return LBM_OK;
+}
+
+int lbm_batch_set_cells_per_thread(lbm_batch* b, int cells_per_thread) {
+    if (!b) return lbm_fail(LBM_ERR_ARG, "lbm_batch_set_cells_per_thread: null batch");
+    if (cells_per_thread != 1 && cells_per_thread != 2 && cells_per_thread != 4)
+        return lbm_fail(LBM_ERR_ARG, "lbm_batch_set_cells_per_thread: %d (1, 2 or 4)", cells_per_thread);
+    b->cpt = cells_per_thread;
     return LBM_OK;
 }
 
@@ -625,6 +650,7 @@ int lbm_batch_get_info(const lbm_batch* b, lbm_batch_info* info) {
     info->threads = b->threads;
     info->blocks = b->blocks;
     info->graph_steps = GRAPH_STEPS;
+    info->cells_per_thread = b->cpt;
     info->stream = b->stream;
     return LBM_OK;
 }

@@ -120,16 +120,17 @@ struct LoopbackHalo {
 // one wavefront per x-row segment of 64 cells, so every population load and
 // store of a wave is 64 consecutive elements.
 // LOOP: the exchange looped back inside the step (LoopbackHalo).
-template <class S, int COLL, bool MACRO, bool HALO, bool LOOP>
+// MX: FP16 MRT with the forward moment transform on the matrix cores.
+template <class S, int COLL, bool MACRO, bool HALO, bool LOOP, bool MX = false>
 __global__ __launch_bounds__(256) void k_cavity_step(const StepArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= a.nx || j >= a.ny) return;
     if constexpr (LOOP)
-        cavity_cell<S, COLL, MACRO, HALO>(a, i, j, k, LoopbackHalo<S>{a, i, j, k});
+        cavity_cell<S, COLL, MACRO, HALO, MX>(a, i, j, k, LoopbackHalo<S>{a, i, j, k});
     else
-        cavity_cell<S, COLL, MACRO, HALO>(a, i, j, k);
+        cavity_cell<S, COLL, MACRO, HALO, MX>(a, i, j, k);
 }
 
 // The split step (halo_overlap): the boundary shell -- every cell on a halo
@@ -193,12 +194,23 @@ constexpr LeavingTable LEAVING = make_leaving();
 // Pack every outgoing region of state `src` into `buf` (packed layout).
 // grid.y = region; grid.x covers the largest region.
 template <class T>
-__global__ __launch_bounds__(256) void k_halo_pack(const T* __restrict__ src, T* __restrict__ buf,
-                                                   const StepArgs a, unsigned mask) {
+__global__ __launch_bounds__(256) void k_halo_pack(const T* __restrict__ src, const T* __restrict__ gsrc,
+                                                   T* __restrict__ buf, const StepArgs a, unsigned mask) {
     const int r = blockIdx.y;
     if (!((mask >> r) & 1u)) return;
     const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     int i, j, k, al;
+    if (r >= GREGION0) {
+        // thermal: the D3Q7 population leaving through face r - GREGION0
+        const int f = r - GREGION0, axis = face_axis(f), hi = f & 1;
+        if (e >= a.hcount[r]) return;
+        if (axis == 0) { i = hi ? a.nx - 1 : 0; j = int(e % a.ny); k = int(e / a.ny); }
+        else if (axis == 1) { j = hi ? a.ny - 1 : 0; i = int(e % a.nx); k = int(e / a.nx); }
+        else { k = hi ? a.nz - 1 : 0; i = int(e % a.nx); j = int(e / a.nx); }
+        const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
+        buf[a.hoff[r] + e] = gsrc[(long long)g_leaving_pop(f) * a.ps + c];
+        return;
+    }
     if (r < NFACE) {
         const int axis = face_axis(r), hi = r & 1;
         const long long plane = axis == 0 ? (long long)a.ny * a.nz
@@ -235,7 +247,7 @@ __global__ __launch_bounds__(256) void k_halo_unpack(const T* __restrict__ recv,
     if (!((mask >> r) & 1u)) return;
     const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (e >= a.hcount[r]) return;
-    const int from = (loopback && r < NFACE) ? (r ^ 1) : r;
+    const int from = !loopback ? r : (r < NFACE ? (r ^ 1) : (r >= GREGION0 ? GREGION0 + ((r - GREGION0) ^ 1) : r));
     halo[a.hoff[r] + e] = recv[a.hoff[from] + e];
 }
 
@@ -401,6 +413,16 @@ struct Table {
          {{k_cavity_step<S, COLL_BGK, true, false, false>, k_cavity_step<S, COLL_BGK, true, false, true>},
           {k_cavity_step<S, COLL_BGK, true, true, false>, k_cavity_step<S, COLL_BGK, true, true, true>}}}};
     static constexpr InitFn init[2] = {k_cavity_init<S, COLL_MRT>, k_cavity_init<S, COLL_BGK>};
+    // FP16 MRT with the matrix-core forward transform: stepx[macro][halo][loop]
+    static constexpr StepFn stepx[2][2][2] = {
+        {{k_cavity_step<StoreF16, COLL_MRT, false, false, false, true>,
+          k_cavity_step<StoreF16, COLL_MRT, false, false, true, true>},
+         {k_cavity_step<StoreF16, COLL_MRT, false, true, false, true>,
+          k_cavity_step<StoreF16, COLL_MRT, false, true, true, true>}},
+        {{k_cavity_step<StoreF16, COLL_MRT, true, false, false, true>,
+          k_cavity_step<StoreF16, COLL_MRT, true, false, true, true>},
+         {k_cavity_step<StoreF16, COLL_MRT, true, true, false, true>,
+          k_cavity_step<StoreF16, COLL_MRT, true, true, true, true>}}};
     static constexpr ShellFn shell[2][2][2] = {
         {{k_cavity_shell<S, COLL_MRT, false, false>, k_cavity_shell<S, COLL_MRT, false, true>},
          {k_cavity_shell<S, COLL_MRT, true, false>, k_cavity_shell<S, COLL_MRT, true, true>}},
@@ -439,7 +461,7 @@ StepArgs make_args(const lbm_ctx* c) {
     a.w = c->macro + 3 * c->n;
     a.halo_faces = c->p.halo_faces;
     a.halo = c->hin;
-    for (int r = 0; r < NREGION; ++r) {
+    for (int r = 0; r < NREGION_ALL; ++r) {
         a.hoff[r] = c->hoff[r];
         a.hcount[r] = c->hcount[r];
     }
@@ -458,38 +480,45 @@ StepFn step_fn(const lbm_ctx* c, bool macro, bool loop) {
     const int halo = c->p.halo_faces != 0 ? 1 : 0;
     switch (c->p.storage) {
     case LBM_STORE_F32: return Table<StoreF32>::step[coll][macro][halo][loop];
-    case LBM_STORE_F16: return Table<StoreF16>::step[coll][macro][halo][loop];
+    case LBM_STORE_F16:
+        if (coll == COLL_MRT && c->p.transform == 2) return Table<StoreF16>::stepx[macro][halo][loop];
+        return Table<StoreF16>::step[coll][macro][halo][loop];
     default: return Table<StoreF64>::step[coll][macro][halo][loop];
     }
 }
 
 long long halo_max_region(const lbm_ctx* c) {
     long long m = 0;
-    for (int r = 0; r < NREGION; ++r) m = c->hcount[r] > m ? c->hcount[r] : m;
+    for (int r = 0; r < NREGION_ALL; ++r) m = c->hcount[r] > m ? c->hcount[r] : m;
     return m;
 }
 
-constexpr unsigned ALL_REGIONS = (1u << NREGION) - 1u;
+// Regions of the context's packed layout: the D3Q19 faces and edges, and for
+// a thermal context the D3Q7 faces.
+int regions(const lbm_ctx* c) { return c->p.thermal ? NREGION_ALL : NREGION; }
+
+constexpr unsigned ALL_REGIONS = (1u << NREGION_ALL) - 1u;
 
 // Launch the pack (dst = send buffer) or unpack (src = receive buffer) kernel
 // for the context's storage type, for the regions in `mask`, on `stream`.
 int launch_pack(lbm_ctx* c, const void* state, void* send, unsigned mask = ALL_REGIONS,
-                hipStream_t stream = nullptr) {
+                hipStream_t stream = nullptr, const void* gstate = nullptr) {
     const StepArgs a = make_args(c);
-    const dim3 grid((unsigned)((halo_max_region(c) + 255) / 256), NREGION);
+    const dim3 grid((unsigned)((halo_max_region(c) + 255) / 256), regions(c));
     if (!stream) stream = c->stream;
+    if (!gstate) gstate = c->p.thermal ? c->gbuf[c->cur] : state;
     switch (c->p.storage) {
     case LBM_STORE_F32:
-        hipLaunchKernelGGL(k_halo_pack<float>, grid, dim3(256), 0, stream, (const float*)state, (float*)send, a,
-                           mask);
+        hipLaunchKernelGGL(k_halo_pack<float>, grid, dim3(256), 0, stream, (const float*)state,
+                           (const float*)gstate, (float*)send, a, mask);
         break;
     case LBM_STORE_F16:
         hipLaunchKernelGGL(k_halo_pack<_Float16>, grid, dim3(256), 0, stream, (const _Float16*)state,
-                           (_Float16*)send, a, mask);
+                           (const _Float16*)gstate, (_Float16*)send, a, mask);
         break;
     default:
-        hipLaunchKernelGGL(k_halo_pack<double>, grid, dim3(256), 0, stream, (const double*)state, (double*)send, a,
-                           mask);
+        hipLaunchKernelGGL(k_halo_pack<double>, grid, dim3(256), 0, stream, (const double*)state,
+                           (const double*)gstate, (double*)send, a, mask);
     }
     return lbm_check_launch();
 }
@@ -497,7 +526,7 @@ int launch_pack(lbm_ctx* c, const void* state, void* send, unsigned mask = ALL_R
 int launch_unpack(lbm_ctx* c, const void* recv, int loopback, unsigned mask = ALL_REGIONS,
                   hipStream_t stream = nullptr) {
     const StepArgs a = make_args(c);
-    const dim3 grid((unsigned)((halo_max_region(c) + 255) / 256), NREGION);
+    const dim3 grid((unsigned)((halo_max_region(c) + 255) / 256), regions(c));
     if (!stream) stream = c->stream;
     switch (c->p.storage) {
     case LBM_STORE_F32:
@@ -541,6 +570,12 @@ void loopback_masks(const lbm_ctx* c, unsigned* pack, unsigned* unpack) {
             pm |= 1u << edge_region(al);
         }
     }
+    if (c->p.thermal)
+        for (int f = 0; f < NFACE; ++f)
+            if ((m >> f) & 1) {
+                um |= 1u << (GREGION0 + f);
+                pm |= 1u << (GREGION0 + (f ^ 1));
+            }
     *pack = pm;
     *unpack = um;
 }
@@ -605,6 +640,18 @@ int fresh_loopback_halo(lbm_ctx* c) {
     c->halo_fresh = true;
     return LBM_OK;
 }
+
+}  // namespace
+
+int halo_loopback_exchange(lbm_ctx* c) {
+    unsigned pmask, umask;
+    loopback_masks(c, &pmask, &umask);
+    int rc = launch_pack(c, c->buf[c->cur], c->hsend, pmask);
+    if (!rc) rc = launch_unpack(c, c->hsend, 1, umask);
+    return rc;
+}
+
+namespace {
 
 // After a looped-back step: the halo it wrote is the one the next step reads.
 void swap_halos(lbm_ctx* c) {
@@ -766,10 +813,9 @@ int lbm_create(const lbm_params* p, lbm_ctx** out) {
         return lbm_fail(LBM_ERR_ARG, "lbm_create: storage %d unknown", p->storage);
     if (p->thermal != 0 && p->thermal != 1) return lbm_fail(LBM_ERR_ARG, "lbm_create: thermal = %d", p->thermal);
     if (p->thermal) {
-        if (p->storage != LBM_STORE_F64)
-            return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_create: the thermal path stores FP64 populations only");
-        if (p->halo_faces || p->halo_loopback)
-            return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_create: the thermal path has no halo exchange");
+        if (p->collision != LBM_MRT)
+            return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_create: the thermal path's flow collision is MRT "
+                                                 "(evolution_f.F90)");
         if (p->u_lid != 0.0)
             return lbm_fail(LBM_ERR_ARG, "lbm_create: the buoyancy cavity has no moving lid (u_lid = %g)", p->u_lid);
         if (p->thermal_axis < -1 || p->thermal_axis > 2)
@@ -781,8 +827,6 @@ int lbm_create(const lbm_params* p, lbm_ctx** out) {
         return lbm_fail(LBM_ERR_ARG, "lbm_create: f16_scale_log2 %d out of [0, 20]", p->f16_scale_log2);
     if (p->transform < 0 || p->transform > 2)
         return lbm_fail(LBM_ERR_ARG, "lbm_create: transform %d (0 library's choice, 1 VALU, 2 MFMA)", p->transform);
-    if (p->transform == 2)
-        return lbm_fail(LBM_ERR_UNSUPPORTED, "lbm_create: no MFMA moment transform in this build");
     // default launch shape: one block per 256-cell x-row run (measured best at
     // 256^3: whole rows per workgroup share their partial edge cache lines)
     const int bx_def = p->nx >= 256 ? 256 : (p->nx + 63) / 64 * 64;
@@ -852,6 +896,15 @@ int lbm_create(const lbm_params* p, lbm_ctx** out) {
             }
             c->hoff[r] = off;
             c->hcount[r] = n;
+            off += n;
+        }
+        // thermal: one D3Q7 population per face plane
+        for (int f = 0; p->thermal && f < NFACE; ++f) {
+            const int axis = face_axis(f);
+            const long long n = axis == 0 ? (long long)p->ny * p->nz
+                                          : (axis == 1 ? (long long)p->nx * p->nz : (long long)p->nx * p->ny);
+            c->hoff[GREGION0 + f] = off;
+            c->hcount[GREGION0 + f] = n;
             off += n;
         }
         c->htotal = off;
@@ -1168,7 +1221,8 @@ int lbm_download_temperature(lbm_ctx* c, double* T) {
 
 int lbm_halo_layout(const lbm_ctx* c, int region, long long* offset, long long* count) {
     if (!c || !offset || !count) return lbm_fail(LBM_ERR_ARG, "lbm_halo_layout: null argument");
-    if (region < 0 || region >= NREGION) return lbm_fail(LBM_ERR_ARG, "lbm_halo_layout: region %d", region);
+    if (region < 0 || region >= regions(c))
+        return lbm_fail(LBM_ERR_ARG, "lbm_halo_layout: region %d (this context has %d)", region, regions(c));
     *offset = c->hoff[region];
     *count = c->hcount[region];
     return LBM_OK;
@@ -1253,6 +1307,9 @@ int lbm_get_info(const lbm_ctx* c, lbm_info* info) {
     info->steps_done = c->itc;
     info->device_bytes = c->device_bytes;
     info->bytes_per_lup = 2.0 * Q * (double)c->elem;
+    if (c->p.thermal)  // + the D3Q7 populations and the stored force (Fz; Fx, Fy too with rotation)
+        info->bytes_per_lup = 2.0 * (Q + Q7) * (double)c->elem +
+                              (c->p.omega_rot != 0.0 ? 6.0 : 2.0) * (double)thermal_force_elem(c);
     info->storage = c->p.storage;
     info->collision = c->p.collision;
     return LBM_OK;

@@ -50,6 +50,7 @@ struct StoreF64 {
     static constexpr bool SHIFT = false;
     static constexpr bool RANDOM = false;
     static constexpr int BYTES = 8;
+    __device__ __forceinline__ static Real val(T v, float) { return v; }
     __device__ __forceinline__ static Real ld(const T* p, float) { return *p; }
     __device__ __forceinline__ static T cvt(Real v, float, unsigned) { return v; }
     __device__ __forceinline__ static void st(T* p, Real v, float, unsigned) { __builtin_nontemporal_store(v, p); }
@@ -61,6 +62,7 @@ struct StoreF32 {
     static constexpr bool SHIFT = true;
     static constexpr bool RANDOM = false;
     static constexpr int BYTES = 4;
+    __device__ __forceinline__ static Real val(T v, float) { return v; }
     __device__ __forceinline__ static Real ld(const T* p, float) { return *p; }
     __device__ __forceinline__ static T cvt(Real v, float, unsigned) { return v; }
     __device__ __forceinline__ static void st(T* p, Real v, float, unsigned) { __builtin_nontemporal_store(v, p); }
@@ -98,12 +100,13 @@ struct StoreF16 {
     static constexpr bool SHIFT = true;
     static constexpr bool RANDOM = true;
     static constexpr int BYTES = 2;
-    __device__ __forceinline__ static Real ld(const T* p, float inv_scale) {
+    __device__ __forceinline__ static Real val(T v, float inv_scale) {
         // fma with +0 rather than a product: one mixed-precision fma
         // (v_fma_mix_f32) instead of a conversion and a multiply; the value
         // is the same (the scale is a power of two)
-        return __builtin_fmaf(static_cast<float>(*p), inv_scale, 0.0f);
+        return __builtin_fmaf(static_cast<float>(v), inv_scale, 0.0f);
     }
+    __device__ __forceinline__ static Real ld(const T* p, float inv_scale) { return val(*p, inv_scale); }
     // the stored value: v scaled, rounded stochastically to FP16
     __device__ __forceinline__ static T cvt(Real v, float scale, unsigned rnd) {
         return to_half_stochastic(v * scale, rnd);
@@ -307,66 +310,76 @@ namespace lbm {
 
 // MRT with the Guo source in moment space (evolution_f.F90:58-129):
 // m* = m - s (m - meq) + (1 - s/2) S, meq in the thermal form (meq(12) with rho).
-// Absolute (unshifted) FP64 arithmetic.
-__device__ __forceinline__ void collide_mrt_forced(double f[Q], double rho, double u, double v, double w,
-                                                   double Fx, double Fy, double Fz, double snu, double sq) {
-    double m[Q], meq[Q], S[Q];
+// R = double, SHIFT = false: absolute FP64 arithmetic in the reference's
+// operation order.  SHIFT = true: f holds f - w and rho = 1 + drho; M w =
+// (1, -11, 3, 0, ...) leaves drho in meq rows 0..2 (the relaxation and the
+// source are unchanged, being differences and forces).
+template <class R, bool SHIFT>
+__device__ __forceinline__ void collide_mrt_forced(R f[Q], R rho, R drho, R u, R v, R w, R Fx, R Fy, R Fz,
+                                                   R snu, R sq) {
+    R m[Q], meq[Q], S[Q];
     mrt_forward(f, m);
-    const double us2 = u * u + v * v + w * w;
-    meq[0] = rho;
-    meq[1] = -11.0 * rho + 19.0 * rho * us2;
-    meq[2] = 3.0 * rho - 11.0 / 2.0 * rho * us2;
+    const R us2 = u * u + v * v + w * w;
+    if constexpr (SHIFT) {
+        meq[0] = drho;
+        meq[1] = R(-11.0) * drho + R(19.0) * rho * us2;
+        meq[2] = R(3.0) * drho - R(11.0 / 2.0) * rho * us2;
+    } else {
+        meq[0] = rho;
+        meq[1] = R(-11.0) * rho + R(19.0) * rho * us2;
+        meq[2] = R(3.0) * rho - R(11.0 / 2.0) * rho * us2;
+    }
     meq[3] = rho * u;
-    meq[4] = -2.0 / 3.0 * meq[3];
+    meq[4] = R(-2.0 / 3.0) * meq[3];
     meq[5] = rho * v;
-    meq[6] = -2.0 / 3.0 * meq[5];
+    meq[6] = R(-2.0 / 3.0) * meq[5];
     meq[7] = rho * w;
-    meq[8] = -2.0 / 3.0 * meq[7];
-    meq[9] = rho * (2.0 * u * u - v * v - w * w);
-    meq[10] = -0.5 * meq[9];
+    meq[8] = R(-2.0 / 3.0) * meq[7];
+    meq[9] = rho * (R(2.0) * u * u - v * v - w * w);
+    meq[10] = R(-0.5) * meq[9];
     meq[11] = rho * (v * v - w * w);
-    meq[12] = -0.5 * meq[11];
+    meq[12] = R(-0.5) * meq[11];
     meq[13] = rho * (u * v);
     meq[14] = rho * (v * w);
     meq[15] = rho * (w * u);
-    meq[16] = meq[17] = meq[18] = 0.0;
-    const double uF = u * Fx + v * Fy + w * Fz;
-    S[0] = 0.0;
-    S[1] = 38.0 * uF;
-    S[2] = -11.0 * uF;
+    meq[16] = meq[17] = meq[18] = R(0.0);
+    const R uF = u * Fx + v * Fy + w * Fz;
+    S[0] = R(0.0);
+    S[1] = R(38.0) * uF;
+    S[2] = R(-11.0) * uF;
     S[3] = Fx;
-    S[4] = -2.0 / 3.0 * Fx;
+    S[4] = R(-2.0 / 3.0) * Fx;
     S[5] = Fy;
-    S[6] = -2.0 / 3.0 * Fy;
+    S[6] = R(-2.0 / 3.0) * Fy;
     S[7] = Fz;
-    S[8] = -2.0 / 3.0 * Fz;
-    S[9] = 4.0 * u * Fx - 2.0 * v * Fy - 2.0 * w * Fz;
-    S[10] = -2.0 * u * Fx + v * Fy + w * Fz;
-    S[11] = 2.0 * v * Fy - 2.0 * w * Fz;
+    S[8] = R(-2.0 / 3.0) * Fz;
+    S[9] = R(4.0) * u * Fx - R(2.0) * v * Fy - R(2.0) * w * Fz;
+    S[10] = R(-2.0) * u * Fx + v * Fy + w * Fz;
+    S[11] = R(2.0) * v * Fy - R(2.0) * w * Fz;
     S[12] = -v * Fy + w * Fz;
     S[13] = u * Fy + v * Fx;
     S[14] = v * Fz + w * Fy;
     S[15] = u * Fz + w * Fx;
-    S[16] = S[17] = S[18] = 0.0;
+    S[16] = S[17] = S[18] = R(0.0);
     sfor<Q>([&](auto K) {
         constexpr int k = decltype(K)::value;
         constexpr int kind = rate_kind<k>();
         if constexpr (kind == 0) m[k] = m[k] + S[k];
-        else if constexpr (kind == 1) m[k] = m[k] - snu * (m[k] - meq[k]) + (1.0 - 0.5 * snu) * S[k];
-        else m[k] = m[k] - sq * (m[k] - meq[k]) + (1.0 - 0.5 * sq) * S[k];
+        else if constexpr (kind == 1) m[k] = m[k] - snu * (m[k] - meq[k]) + (R(1.0) - R(0.5) * snu) * S[k];
+        else m[k] = m[k] - sq * (m[k] - meq[k]) + (R(1.0) - R(0.5) * sq) * S[k];
     });
     mrt_inverse(m, f);
 }
 
 // D3Q7 MRT for the temperature (evolution_g.F90:1-57): n = N g,
 // neq = (T, T u, T v, T w, T paraA, 0, 0), rates (0, Qd x3, Qnu x3),
-// g_post = N^-1 n*, N^-1 = N^T diag(1/|row|^2).
-__device__ __forceinline__ void collide_d3q7(double g[Q7], double T, double u, double v, double w,
-                                             double para_a, double qd, double qnu) {
-    double n[Q7];
+// g_post = N^-1 n*, N^-1 = N^T diag(1/|row|^2).  g is stored unshifted.
+template <class R>
+__device__ __forceinline__ void collide_d3q7(R g[Q7], R T, R u, R v, R w, R para_a, R qd, R qnu) {
+    R n[Q7];
     sfor<Q7>([&](auto K) {
         constexpr int k = decltype(K)::value;
-        double acc = -0.0;
+        R acc = R(-0.0);
         sfor<Q7>([&](auto A) {
             constexpr int a = decltype(A)::value;
             constexpr int c = basis7(k, a);
@@ -374,7 +387,7 @@ __device__ __forceinline__ void collide_d3q7(double g[Q7], double T, double u, d
         });
         n[k] = acc;
     });
-    const double neq[Q7] = {T, T * u, T * v, T * w, T * para_a, 0.0, 0.0};
+    const R neq[Q7] = {T, T * u, T * v, T * w, T * para_a, R(0.0), R(0.0)};
     sfor<Q7>([&](auto K) {
         constexpr int k = decltype(K)::value;
         if constexpr (k >= 1 && k <= 3) n[k] = n[k] - qd * (n[k] - neq[k]);
@@ -382,11 +395,11 @@ __device__ __forceinline__ void collide_d3q7(double g[Q7], double T, double u, d
     });
     sfor<Q7>([&](auto A) {
         constexpr int a = decltype(A)::value;
-        double acc = -0.0;
+        R acc = R(-0.0);
         sfor<Q7>([&](auto K) {
             constexpr int k = decltype(K)::value;
             constexpr int c = basis7(k, a);
-            if constexpr (c != 0) acc = acc + (double(c) / double(row_norm2_7(k))) * n[k];
+            if constexpr (c != 0) acc = acc + R(double(c) / double(row_norm2_7(k))) * n[k];
         });
         g[a] = acc;
     });

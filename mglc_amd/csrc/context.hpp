@@ -49,13 +49,13 @@ struct lbm_ctx {
     unsigned step_lds_halo = 0;  // the same for the kernels that read a halo (and the thermal step)
     float scale = 1.0f, inv_scale = 1.0f;
     long long device_bytes = 0;
-    long long hoff[lbm::NREGION] = {}, hcount[lbm::NREGION] = {}, htotal = 0;
+    long long hoff[lbm::NREGION_ALL] = {}, hcount[lbm::NREGION_ALL] = {}, htotal = 0;  // D3Q7 regions: thermal only
     void* hin = nullptr;         // incoming halo regions (packed layout)
     void* hsend = nullptr;       // loopback send buffer
     // buoyancy-driven cavity (thermal = 1): D3Q7 temperature populations,
     // the body force of the last collision, T and its previous check value
     void* gbuf[2] = {nullptr, nullptr};  // 7 * ps each
-    double* force = nullptr;     // Fx, Fy, Fz of the last collision (3 * ps), in place
+    void* force = nullptr;       // Fx, Fy, Fz of the last collision (3 * ps; FP64 with FP64 storage, else FP32)
     double* temp = nullptr;      // T (n)
     double* prev_t = nullptr;    // wp, Tp for the thermal check (2 * n)
     ParticleState* part = nullptr;  // moving spheres, when attached
@@ -69,7 +69,12 @@ struct lbm_ctx {
 };
 
 // Thermal path (thermal.hip).
+// The looped-back exchange of every region the next step reads (pack from the
+// current state, unpack into the opposite faces' regions): the thermal path
+// runs it between steps.
+int halo_loopback_exchange(lbm_ctx* c);
 int thermal_alloc(lbm_ctx* c);
+size_t thermal_force_elem(const lbm_ctx* c);  // bytes of one stored force component
 void thermal_release(lbm_ctx* c);
 int thermal_load(lbm_ctx* c, const double* d_rho, const double* d_u, const double* d_v, const double* d_w,
                  const double* d_T);

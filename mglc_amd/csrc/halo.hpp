@@ -12,6 +12,11 @@ namespace lbm {
 
 constexpr int NFACE = 6;    // XM, XP, YM, YP, ZM, ZP
 constexpr int NREGION = 18; // faces, then edges 7..18
+// Thermal contexts add one region per face: the D3Q7 population crossing it
+// (message_exchange.F90: g_post(1) of plane nx to +x, ..., g_post(6) of plane
+// 1 to -z), packed as a face plane like the D3Q19 faces' slots.
+constexpr int GREGION0 = NREGION;         // regions 18..23 = faces XM..ZP
+constexpr int NREGION_ALL = NREGION + NFACE;
 
 constexpr int face_axis(int f) { return f / 2; }
 constexpr int face_sign(int f) { return (f % 2) ? 1 : -1; }
@@ -45,5 +50,8 @@ constexpr int entry_face(int a, int axis) {
 constexpr int edge_free_axis(int a) { return EX[a] == 0 ? 0 : (EY[a] == 0 ? 1 : 2); }
 
 constexpr int edge_region(int a) { return NFACE + (a - 7); }
+
+// D3Q7 population leaving through face f (1 +x, 2 -x, 3 +y, 4 -y, 5 +z, 6 -z).
+constexpr int g_leaving_pop(int f) { return 1 + 2 * face_axis(f) + ((f & 1) ? 0 : 1); }
 
 }  // namespace lbm

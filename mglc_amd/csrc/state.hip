@@ -14,7 +14,7 @@
 //   macro    4 * n doubles (rho, u, v, w)
 //   prev     2 * n doubles (up, vp of the residual)
 //   halo     htotal elements (incoming halo regions), when halo faces are set
-//   thermal  g (Q7 * ps doubles), force (3 * ps), T (n), wp/Tp (2 * n)
+//   thermal  g (Q7 * ps elements), force (3 * ps, FP64 or FP32), T (n), wp/Tp (2 * n)
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -64,8 +64,8 @@ int sections(lbm_ctx* c, Section* out) {
     out[k++] = {c->prev, sizeof(double) * 2 * n};
     if (c->p.halo_faces) out[k++] = {c->hin, c->elem * (size_t)c->htotal};
     if (c->p.thermal) {
-        out[k++] = {c->gbuf[c->cur], sizeof(double) * (size_t)c->ps * lbm::Q7};
-        out[k++] = {c->force, sizeof(double) * 3 * (size_t)c->ps};
+        out[k++] = {c->gbuf[c->cur], c->elem * (size_t)c->ps * lbm::Q7};
+        out[k++] = {c->force, thermal_force_elem(c) * 3 * (size_t)c->ps};
         out[k++] = {c->temp, sizeof(double) * n};
         out[k++] = {c->prev_t, sizeof(double) * 2 * n};
     }

@@ -6,9 +6,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "collide.hpp"
 #include "context.hpp"
 #include "halo.hpp"
+#include "mfma.hpp"
 
 namespace lbm {
 
@@ -30,8 +33,8 @@ struct StepArgs {
     double* w;
     int halo_faces;     // bit f: face f reads the halo instead of bouncing back
     const void* halo;   // incoming regions, packed layout (HALO kernels only)
-    long long hoff[NREGION];    // element offset of each region in the packed layout
-    long long hcount[NREGION];  // elements in each region
+    long long hoff[NREGION_ALL];    // element offset of each region in the packed layout
+    long long hcount[NREGION_ALL];  // elements in each region (D3Q7 regions: thermal contexts only)
     unsigned seed;              // step number: drives the stochastic rounding of FP16 stores
     void* halo_out;             // loopback: the halo of the next step, written by the step itself
     unsigned loop_mask;         // loopback: the regions of halo_out the next step reads (bit = region)
@@ -117,13 +120,80 @@ __device__ __forceinline__ typename S::Real pull_outside(const Args& a, const ty
     return val;
 }
 
+// One FP16-stored MRT cell of a whole wavefront away from the y and z faces,
+// with the forward moment transform on the matrix cores: the populations stay
+// in their stored FP16 form (the transform takes them exactly), the moments
+// and the relaxation are FP32 in the stored scale 2^k, and the inverse
+// transform (VALU) gives the scaled post-collision values that the
+// stochastic rounding stores directly.  Same pulls, walls, stores and hook as
+// the VALU path; the forward sums run in the matrix cores' order.
+template <bool MACRO, bool HALO, class Args, class Post>
+__device__ __forceinline__ void mfma_cell(const Args& a, const _Float16* src, _Float16* dst, int i, int j, int k,
+                                          int c, const Post& post) {
+    using S = StoreF16;
+    _Float16 h[Q];
+    sfor<Q>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        const long long off = (long long)EX[al] + (long long)EY[al] * a.px + (long long)EZ[al] * a.pxy;
+        h[al] = src[(long long)al * a.ps + c - off];
+    });
+    if (i == 0 || i == a.nx - 1) {
+        // an x-face lane: halfway bounce-back or the halo (no lid here: not
+        // the top layer), back in the stored form (exact: a power-of-two scale)
+        sfor<Q>([&](auto A) {
+            constexpr int al = decltype(A)::value;
+            if constexpr (EX[al] != 0) {
+                if ((unsigned)(i - EX[al]) >= (unsigned)a.nx)
+                    h[al] = (_Float16)(pull_outside<S, al, HALO>(a, src, i, j, k, c, false, 0.0) * a.scale);
+            }
+        });
+    }
+    const int lane = __lane_id();
+    float m[Q];
+    mrt_forward_mfma(h, m, lane);
+    Moments<float, true> q;
+    q.drho = m[0] * a.inv_scale;
+    q.rho = 1.0f + q.drho;
+    q.u = m[3] * a.inv_scale / q.rho;
+    q.v = m[5] * a.inv_scale / q.rho;
+    q.w = m[7] * a.inv_scale / q.rho;
+    if constexpr (MACRO) {
+        const long long mm = i + (long long)a.nx * (j + (long long)a.ny * k);
+        a.rho[mm] = 1.0 + double(q.drho);
+        a.u[mm] = double(q.u);
+        a.v[mm] = double(q.v);
+        a.w[mm] = double(q.w);
+    }
+    float meq[Q];
+    mrt_meq<float, true>(q, a.meq12_rho != 0, meq);
+    const float snu = float(a.snu), sq = float(a.sq);
+    sfor<Q>([&](auto K) {
+        constexpr int kk = decltype(K)::value;
+        constexpr int kind = rate_kind<kk>();
+        if constexpr (kind == 1) m[kk] = m[kk] - snu * (m[kk] - a.scale * meq[kk]);
+        else if constexpr (kind == 2) m[kk] = m[kk] - sq * (m[kk] - a.scale * meq[kk]);
+    });
+    float f[Q];
+    mrt_inverse(m, f);
+    const unsigned hr = cell_random<S>(c, a.seed);
+    _Float16 stored[Q];
+    sfor<Q>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        stored[al] = to_half_stochastic(f[al], store_random<S, al>(hr));
+        __builtin_nontemporal_store(stored[al], dst + (long long)al * a.ps + c);
+    });
+    if (post.active()) post.store(stored);
+}
+
 // The update of cell (i, j, k): pull the 19 populations (bounce-back or halo
 // where the pull leaves the domain), macro, collision, store.  `post` sees
 // the stored values (post.store(values)), e.g. to write them to a halo too,
 // after the last store: the collision and its stores stay one straight-line
 // block in every kernel, so the compiler contracts the arithmetic the same
 // way and every kernel built on this function produces the same bits.
-template <class S, int COLL, bool MACRO, bool HALO, class Args, class Post = NoPost>
+// MX: the forward MRT moment transform of whole FP16 wavefronts away from the
+// y and z faces on the matrix cores (mfma.hpp); other waves use the VALU path.
+template <class S, int COLL, bool MACRO, bool HALO, bool MX = false, class Args, class Post = NoPost>
 __device__ __forceinline__ void cavity_cell(const Args& a, int i, int j, int k, const Post& post = Post{}) {
     using T = typename S::T;
     using R = typename S::Real;
@@ -136,25 +206,54 @@ __device__ __forceinline__ void cavity_cell(const Args& a, int i, int j, int k, 
     if (top) rho_lid = a.lid_rho[lid_idx];
 
     R f[Q];
-    sfor<Q>([&](auto A) {
-        constexpr int al = decltype(A)::value;
-        constexpr int ex = EX[al], ey = EY[al], ez = EZ[al];
-        const T* base = src + (long long)al * a.ps;
-        if constexpr (ex == 0 && ey == 0 && ez == 0) {
-            f[al] = S::ld(base + c, a.inv_scale);
-        } else {
-            bool in = true;
-            if constexpr (ex != 0) in = in && (unsigned)(i - ex) < (unsigned)a.nx;
-            if constexpr (ey != 0) in = in && (unsigned)(j - ey) < (unsigned)a.ny;
-            if constexpr (ez != 0) in = in && (unsigned)(k - ez) < (unsigned)a.nz;
-            if (in) {
-                const long long off = (long long)ex + (long long)ey * a.px + (long long)ez * a.pxy;
-                f[al] = S::ld(base + (c - off), a.inv_scale);
-            } else {
-                f[al] = pull_outside<S, al, HALO>(a, src, i, j, k, c, top, rho_lid);
+    if ((unsigned)(j - 1) < (unsigned)(a.ny - 2) && (unsigned)(k - 1) < (unsigned)(a.nz - 2)) {
+        // Not on a y or z face: every pull from a neighbour row and plane that
+        // exists, so every pull's address lies inside its population's plane
+        // -- also for a lane on an x face, whose pull then reads a
+        // neighbouring row's element and is replaced below.  No per-pull test.
+        if constexpr (MX) {
+            static_assert(std::is_same_v<S, StoreF16> && COLL == COLL_MRT,
+                          "matrix-core transforms: FP16-stored MRT");
+            if (__builtin_amdgcn_read_exec() == ~0ull) {
+                mfma_cell<MACRO, HALO>(a, src, dst, i, j, k, c, post);
+                return;
             }
         }
-    });
+        sfor<Q>([&](auto A) {
+            constexpr int al = decltype(A)::value;
+            const long long off = (long long)EX[al] + (long long)EY[al] * a.px + (long long)EZ[al] * a.pxy;
+            f[al] = S::ld(src + ((long long)al * a.ps + c - off), a.inv_scale);
+        });
+        if (i == 0 || i == a.nx - 1) {
+            sfor<Q>([&](auto A) {
+                constexpr int al = decltype(A)::value;
+                if constexpr (EX[al] != 0) {
+                    if ((unsigned)(i - EX[al]) >= (unsigned)a.nx)
+                        f[al] = pull_outside<S, al, HALO>(a, src, i, j, k, c, top, rho_lid);
+                }
+            });
+        }
+    } else {
+        sfor<Q>([&](auto A) {
+            constexpr int al = decltype(A)::value;
+            constexpr int ex = EX[al], ey = EY[al], ez = EZ[al];
+            const T* base = src + (long long)al * a.ps;
+            if constexpr (ex == 0 && ey == 0 && ez == 0) {
+                f[al] = S::ld(base + c, a.inv_scale);
+            } else {
+                bool in = true;
+                if constexpr (ex != 0) in = in && (unsigned)(i - ex) < (unsigned)a.nx;
+                if constexpr (ey != 0) in = in && (unsigned)(j - ey) < (unsigned)a.ny;
+                if constexpr (ez != 0) in = in && (unsigned)(k - ez) < (unsigned)a.nz;
+                if (in) {
+                    const long long off = (long long)ex + (long long)ey * a.px + (long long)ez * a.pxy;
+                    f[al] = S::ld(base + (c - off), a.inv_scale);
+                } else {
+                    f[al] = pull_outside<S, al, HALO>(a, src, i, j, k, c, top, rho_lid);
+                }
+            }
+        });
+    }
 
     const Moments<R, S::SHIFT> q = moments<R, S::SHIFT>(f);
     if (top) a.lid_rho[lid_idx] = S::SHIFT ? 1.0 + double(q.drho) : double(q.rho);

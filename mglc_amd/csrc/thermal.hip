@@ -19,20 +19,23 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "collide.hpp"
 #include "context.hpp"
+#include "halo.hpp"
+#include "step.hpp"
 
 using namespace lbm;
 
 namespace {
 
 struct ThermalArgs {
-    const double* fsrc;
-    double* fdst;
-    const double* gsrc;
-    double* gdst;
-    double* force;  // Fx | Fy | Fz, each ps long
+    const void* fsrc;
+    void* fdst;
+    const void* gsrc;
+    void* gdst;
+    void* force;  // Fx | Fy | Fz, each ps long (FP64 with FP64 storage, else FP32)
     int nx, ny, nz, px;
     long long pxy, ps;
     double snu, sq;           // flow rates (collision.f90:94-112 assignment)
@@ -40,41 +43,73 @@ struct ThermalArgs {
     double g_beta, omega_rot, t_ref;
     double t_hot, t_cold, cw; // held walls; cw = (6 + paraA) / 21
     int taxis;                // axis of the held walls (0 x, 1 y, 2 z), -1 none
+    float scale, inv_scale;   // FP16 storage: populations held as value * 2^k
+    unsigned seed;            // step number: drives the stochastic rounding of FP16 stores
     double *rho, *u, *v, *w, *T;
+    // subdomain halo (the members the isothermal pull from outside reads):
+    // faces in halo_faces take their incoming populations from `halo`
+    int halo_faces;
+    const void* halo;
+    long long hoff[NREGION_ALL];
+    double u_lid;             // 0: no moving lid here
 };
 
-template <bool ROT, bool MACRO>
+// The stored body force: FP64 beside FP64 populations, else FP32 (the force
+// is ~g beta ~ 1e-8, below FP16's normal range).
+template <class S>
+using ForceT = std::conditional_t<std::is_same_v<S, StoreF64>, double, float>;
+
+// One step of the buoyancy-driven cavity.  S = StoreF64: FP64 arithmetic in
+// the reference's order.  StoreF32 / StoreF16: f stored as f - w (FP16
+// scaled by 2^k with stochastic rounding), g stored unshifted (scaled the
+// same way in FP16), the force in FP32, all arithmetic in FP32; the macro
+// fields are written as FP64 for the FP64 reductions.
+template <class S, bool ROT, bool MACRO, bool HALO>
 __global__ __launch_bounds__(256) void k_thermal_step(const ThermalArgs a) {
+    using ST = typename S::T;
+    using R = typename S::Real;
+    using FT = ForceT<S>;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= a.nx || j >= a.ny) return;
     const int c = i + a.px * (j + a.ny * k);
+    const ST* fsrc = static_cast<const ST*>(a.fsrc);
+    const ST* gsrc = static_cast<const ST*>(a.gsrc);
+    FT* force = static_cast<FT*>(a.force);
 
     // streaming + bounceback (evolution_f.F90:227-343): no-slip on all walls
-    double f[Q];
+    R f[Q];
     sfor<Q>([&](auto A) {
         constexpr int al = decltype(A)::value;
         constexpr int ex = EX[al], ey = EY[al], ez = EZ[al];
         if constexpr (al == 0) {
-            f[0] = a.fsrc[c];
+            f[0] = S::ld(fsrc + c, a.inv_scale);
         } else {
             bool in = true;
             if constexpr (ex != 0) in = in && (unsigned)(i - ex) < (unsigned)a.nx;
             if constexpr (ey != 0) in = in && (unsigned)(j - ey) < (unsigned)a.ny;
             if constexpr (ez != 0) in = in && (unsigned)(k - ez) < (unsigned)a.nz;
             const long long off = (long long)ex + (long long)ey * a.px + (long long)ez * a.pxy;
-            f[al] = in ? a.fsrc[al * a.ps + c - off] : a.fsrc[OPP[al] * a.ps + c];
+            if constexpr (HALO) {
+                // from the halo where every crossed face borders a neighbour,
+                // else bounce-back (no lid: top = false)
+                f[al] = in ? S::ld(fsrc + (al * a.ps + c - off), a.inv_scale)
+                           : pull_outside<S, al, true>(a, fsrc, i, j, k, c, false, 0.0);
+            } else {
+                f[al] = in ? S::ld(fsrc + (al * a.ps + c - off), a.inv_scale)
+                           : S::ld(fsrc + (OPP[al] * a.ps + c), a.inv_scale);
+            }
         }
     });
     // streamingT + bouncebackT (evolution_g.F90:60-197): adiabatic walls bounce
     // back; held walls: g(a) = -g_post(abar) + (6 + paraA)/21 Twall
-    double g[Q7];
+    R g[Q7];
     sfor<Q7>([&](auto A) {
         constexpr int al = decltype(A)::value;
         constexpr int ex = EX[al], ey = EY[al], ez = EZ[al];
         if constexpr (al == 0) {
-            g[0] = a.gsrc[c];
+            g[0] = S::ld(gsrc + c, a.inv_scale);
         } else {
             constexpr int axis = ex != 0 ? 0 : (ey != 0 ? 1 : 2);
             constexpr int e = ex + ey + ez;
@@ -82,70 +117,85 @@ __global__ __launch_bounds__(256) void k_thermal_step(const ThermalArgs a) {
             const int nq = axis == 0 ? a.nx : (axis == 1 ? a.ny : a.nz);
             const bool in = (unsigned)(q - e) < (unsigned)nq;
             const long long off = (long long)ex + (long long)ey * a.px + (long long)ez * a.pxy;
+            constexpr int face = e > 0 ? 2 * axis : 2 * axis + 1;  // the face the pull enters through
             if (in) {
-                g[al] = a.gsrc[al * a.ps + c - off];
+                g[al] = S::ld(gsrc + (al * a.ps + c - off), a.inv_scale);
+            } else if (HALO && ((a.halo_faces >> face) & 1)) {
+                // what arrives through `face`: the neighbour's D3Q7 population al
+                const long long p = axis == 0 ? j + (long long)a.ny * k
+                                              : (axis == 1 ? i + (long long)a.nx * k : i + (long long)a.nx * j);
+                g[al] = S::ld(static_cast<const ST*>(a.halo) + (a.hoff[GREGION0 + face] + p), a.inv_scale);
             } else {
-                const double bb = a.gsrc[OPP7[al] * a.ps + c];
-                g[al] = (a.taxis == axis) ? -bb + a.cw * (e > 0 ? a.t_hot : a.t_cold) : bb;
+                const R bb = S::ld(gsrc + (OPP7[al] * a.ps + c), a.inv_scale);
+                g[al] = (a.taxis == axis) ? -bb + R(a.cw) * R(e > 0 ? a.t_hot : a.t_cold) : bb;
             }
         }
     });
 
     // macro (evolution_f.F90:346-370) with the force of the last collision
-    const double Fz0 = a.force[2 * a.ps + c];
-    const double Fx0 = ROT ? a.force[c] : 0.0;
-    const double Fy0 = ROT ? a.force[a.ps + c] : 0.0;
-    double rho = f[0], jx = -0.0, jy = -0.0, jz = -0.0;
+    const R Fz0 = R(force[2 * a.ps + c]);
+    const R Fx0 = ROT ? R(force[c]) : R(0.0);
+    const R Fy0 = ROT ? R(force[a.ps + c]) : R(0.0);
+    R s = f[0], jx = R(-0.0), jy = R(-0.0), jz = R(-0.0);
     sfor<Q - 1>([&](auto Am1) {
         constexpr int al = decltype(Am1)::value + 1;
-        rho = rho + f[al];
+        s = s + f[al];
         if constexpr (EX[al] != 0) jx = mac<EX[al]>(jx, f[al]);
         if constexpr (EY[al] != 0) jy = mac<EY[al]>(jy, f[al]);
         if constexpr (EZ[al] != 0) jz = mac<EZ[al]>(jz, f[al]);
     });
-    const double u = (jx + 0.5 * Fx0) / rho;
-    const double v = (jy + 0.5 * Fy0) / rho;
-    const double w = (jz + 0.5 * Fz0) / rho;
+    const R rho = S::SHIFT ? R(1.0) + s : s;
+    const R drho = S::SHIFT ? s : s - R(1.0);
+    const R u = (jx + R(0.5) * Fx0) / rho;
+    const R v = (jy + R(0.5) * Fy0) / rho;
+    const R w = (jz + R(0.5) * Fz0) / rho;
     // macroT (evolution_g.F90:202-218)
-    double T = g[0];
+    R T = g[0];
     sfor<Q7 - 1>([&](auto Am1) { T = T + g[decltype(Am1)::value + 1]; });
     if constexpr (MACRO) {
         const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
-        a.rho[m] = rho;
-        a.u[m] = u;
-        a.v[m] = v;
-        a.w[m] = w;
-        a.T[m] = T;
+        a.rho[m] = S::SHIFT ? 1.0 + double(drho) : double(rho);
+        a.u[m] = double(u);
+        a.v[m] = double(v);
+        a.w[m] = double(w);
+        a.T[m] = double(T);
     }
     // next collision: force from this step's fields (evolution_f.F90:103-105)
-    const double Fx = ROT ? -2.0 * rho * v * a.omega_rot : 0.0;
-    const double Fy = ROT ? 2.0 * rho * u * a.omega_rot : 0.0;
-    const double Fz = rho * a.g_beta * (T - a.t_ref);
-    collide_mrt_forced(f, rho, u, v, w, Fx, Fy, Fz, a.snu, a.sq);
-    collide_d3q7(g, T, u, v, w, a.para_a, a.qd, a.qnu);
+    const R Fx = ROT ? R(-2.0) * rho * v * R(a.omega_rot) : R(0.0);
+    const R Fy = ROT ? R(2.0) * rho * u * R(a.omega_rot) : R(0.0);
+    const R Fz = rho * R(a.g_beta) * (T - R(a.t_ref));
+    collide_mrt_forced<R, S::SHIFT>(f, rho, drho, u, v, w, Fx, Fy, Fz, R(a.snu), R(a.sq));
+    collide_d3q7<R>(g, T, u, v, w, R(a.para_a), R(a.qd), R(a.qnu));
+    ST* fdst = static_cast<ST*>(a.fdst);
+    ST* gdst = static_cast<ST*>(a.gdst);
+    const unsigned h = cell_random<S>(c, a.seed);
     sfor<Q>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        __builtin_nontemporal_store(f[al], a.fdst + al * a.ps + c);
+        S::st(fdst + (al * a.ps + c), f[al], a.scale, store_random<S, al>(h));
     });
     sfor<Q7>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        __builtin_nontemporal_store(g[al], a.gdst + al * a.ps + c);
+        S::st(gdst + (al * a.ps + c), g[al], a.scale, store_random<S, Q + al>(h));
     });
-    a.force[2 * a.ps + c] = Fz;
+    force[2 * a.ps + c] = FT(Fz);
     if constexpr (ROT) {
-        a.force[c] = Fx;
-        a.force[a.ps + c] = Fy;
+        force[c] = FT(Fx);
+        force[a.ps + c] = FT(Fy);
     }
 }
 
 // Equilibrium start (initial.F90:172-182) and the first collision from the
 // given fields.  rho_in.. are compact nx*ny*nz arrays (also the macro output).
-template <bool ROT>
+// The equilibria are formed in FP64 and the collision runs in the storage's
+// arithmetic, as the step does.
+template <class S, bool ROT>
 __global__ __launch_bounds__(256) void k_thermal_init(const ThermalArgs a, const double* __restrict__ rho_in,
                                                       const double* __restrict__ u_in,
                                                       const double* __restrict__ v_in,
                                                       const double* __restrict__ w_in,
                                                       const double* __restrict__ T_in) {
+    using R = typename S::Real;
+    using FT = ForceT<S>;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
@@ -154,34 +204,41 @@ __global__ __launch_bounds__(256) void k_thermal_init(const ThermalArgs a, const
     const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
     const double rho = rho_in[m], u = u_in[m], v = v_in[m], w = w_in[m], T = T_in[m];
     const double us2 = u * u + v * v + w * w;
-    double f[Q], g[Q7];
+    R f[Q], g[Q7];
     sfor<Q>([&](auto A) {
         constexpr int al = decltype(A)::value;
         const double un = u * EX[al] + v * EY[al] + w * EZ[al];
-        f[al] = rho * weight(al) * (1.0 + 3.0 * un + 4.5 * un * un - 1.5 * us2);
+        const double feq = rho * weight(al) * (1.0 + 3.0 * un + 4.5 * un * un - 1.5 * us2);
+        f[al] = S::SHIFT ? R(feq - weight(al)) : R(feq);
     });
     const double wt0 = (1.0 - a.para_a) / 7.0, wt1 = (a.para_a + 6.0) / 42.0;
     sfor<Q7>([&](auto A) {
         constexpr int al = decltype(A)::value;
         const double un = u * EX[al] + v * EY[al] + w * EZ[al];
-        g[al] = (al == 0 ? wt0 : wt1) * T * (1.0 + 21.0 / (6.0 + a.para_a) * un);
+        g[al] = R((al == 0 ? wt0 : wt1) * T * (1.0 + 21.0 / (6.0 + a.para_a) * un));
     });
     const double Fx = -2.0 * rho * v * a.omega_rot;
     const double Fy = 2.0 * rho * u * a.omega_rot;
     const double Fz = rho * a.g_beta * (T - a.t_ref);
-    collide_mrt_forced(f, rho, u, v, w, Fx, Fy, Fz, a.snu, a.sq);
-    collide_d3q7(g, T, u, v, w, a.para_a, a.qd, a.qnu);
+    collide_mrt_forced<R, S::SHIFT>(f, R(rho), R(rho - 1.0), R(u), R(v), R(w), R(Fx), R(Fy), R(Fz), R(a.snu),
+                                    R(a.sq));
+    collide_d3q7<R>(g, R(T), R(u), R(v), R(w), R(a.para_a), R(a.qd), R(a.qnu));
+    using Tst = typename S::T;
+    Tst* fdst = static_cast<Tst*>(a.fdst);
+    Tst* gdst = static_cast<Tst*>(a.gdst);
+    FT* force = static_cast<FT*>(a.force);
+    const unsigned h = cell_random<S>(c, a.seed);
     sfor<Q>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        a.fdst[al * a.ps + c] = f[al];
+        fdst[al * a.ps + c] = S::cvt(f[al], a.scale, store_random<S, al>(h));
     });
     sfor<Q7>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        a.gdst[al * a.ps + c] = g[al];
+        gdst[al * a.ps + c] = S::cvt(g[al], a.scale, store_random<S, Q + al>(h));
     });
-    a.force[c] = Fx;
-    a.force[a.ps + c] = Fy;
-    a.force[2 * a.ps + c] = Fz;
+    force[c] = FT(Fx);
+    force[a.ps + c] = FT(Fy);
+    force[2 * a.ps + c] = FT(Fz);
     a.rho[m] = rho;
     a.u[m] = u;
     a.v[m] = v;
@@ -286,12 +343,18 @@ ThermalArgs thermal_args(const lbm_ctx* c) {
     a.t_cold = c->p.t_cold;
     a.cw = (6.0 + c->p.para_a) / 21.0;
     a.taxis = c->p.thermal_axis;
+    a.scale = c->scale;
+    a.inv_scale = c->inv_scale;
     a.force = c->force;
     a.rho = c->macro;
     a.u = c->macro + c->n;
     a.v = c->macro + 2 * c->n;
     a.w = c->macro + 3 * c->n;
     a.T = c->temp;
+    a.halo_faces = c->p.halo_faces;
+    a.halo = c->hin;
+    for (int r = 0; r < NREGION_ALL; ++r) a.hoff[r] = c->hoff[r];
+    a.u_lid = 0.0;
     return a;
 }
 
@@ -315,14 +378,17 @@ int finish_sums(lbm_ctx* c, double* out) {
 
 }  // namespace
 
+size_t thermal_force_elem(const lbm_ctx* c) { return c->p.storage == LBM_STORE_F64 ? sizeof(double) : sizeof(float); }
+
 int thermal_alloc(lbm_ctx* c) {
-    const size_t g_bytes = (size_t)c->ps * Q7 * sizeof(double);
+    const size_t g_bytes = (size_t)c->ps * Q7 * c->elem;
+    const size_t f_bytes = 3 * (size_t)c->ps * thermal_force_elem(c);
     HIP_TRY(hipMalloc(&c->gbuf[0], g_bytes));
     HIP_TRY(hipMalloc(&c->gbuf[1], g_bytes));
-    HIP_TRY(hipMalloc(&c->force, 3 * (size_t)c->ps * sizeof(double)));
+    HIP_TRY(hipMalloc(&c->force, f_bytes));
     HIP_TRY(hipMalloc(&c->temp, (size_t)c->n * sizeof(double)));
     HIP_TRY(hipMalloc(&c->prev_t, 2 * (size_t)c->n * sizeof(double)));
-    c->device_bytes += (long long)(2 * g_bytes + (3 * c->ps + 3 * c->n) * sizeof(double));
+    c->device_bytes += (long long)(2 * g_bytes + f_bytes + 3 * c->n * sizeof(double));
     return LBM_OK;
 }
 
@@ -334,33 +400,72 @@ void thermal_release(lbm_ctx* c) {
     if (c->prev_t) (void)hipFree(c->prev_t);
 }
 
+namespace {
+
+using ThermalFn = void (*)(const ThermalArgs);
+using ThermalInitFn = void (*)(const ThermalArgs, const double*, const double*, const double*, const double*,
+                               const double*);
+
+template <class S>
+struct ThermalTable {
+    static constexpr ThermalFn step[2][2][2] = {
+        {{k_thermal_step<S, false, false, false>, k_thermal_step<S, false, false, true>},
+         {k_thermal_step<S, false, true, false>, k_thermal_step<S, false, true, true>}},
+        {{k_thermal_step<S, true, false, false>, k_thermal_step<S, true, false, true>},
+         {k_thermal_step<S, true, true, false>, k_thermal_step<S, true, true, true>}}};
+    static constexpr ThermalInitFn init[2] = {k_thermal_init<S, false>, k_thermal_init<S, true>};
+};
+
+ThermalFn thermal_step_fn(const lbm_ctx* c, bool rot, bool macro, bool halo) {
+    switch (c->p.storage) {
+    case LBM_STORE_F32: return ThermalTable<StoreF32>::step[rot][macro][halo];
+    case LBM_STORE_F16: return ThermalTable<StoreF16>::step[rot][macro][halo];
+    default: return ThermalTable<StoreF64>::step[rot][macro][halo];
+    }
+}
+
+ThermalInitFn thermal_init_fn(const lbm_ctx* c, bool rot) {
+    switch (c->p.storage) {
+    case LBM_STORE_F32: return ThermalTable<StoreF32>::init[rot];
+    case LBM_STORE_F16: return ThermalTable<StoreF16>::init[rot];
+    default: return ThermalTable<StoreF64>::init[rot];
+    }
+}
+
+}  // namespace
+
 int thermal_load(lbm_ctx* c, const double* d_rho, const double* d_u, const double* d_v, const double* d_w,
                  const double* d_T) {
     ThermalArgs a = thermal_args(c);
-    a.fdst = static_cast<double*>(c->buf[0]);
-    a.gdst = static_cast<double*>(c->gbuf[0]);
-    if (c->p.omega_rot != 0.0)
-        hipLaunchKernelGGL(k_thermal_init<true>, thermal_grid(c), c->block, 0, c->stream, a, d_rho, d_u, d_v, d_w,
-                           d_T);
-    else
-        hipLaunchKernelGGL(k_thermal_init<false>, thermal_grid(c), c->block, 0, c->stream, a, d_rho, d_u, d_v,
-                           d_w, d_T);
+    a.fdst = c->buf[0];
+    a.gdst = c->gbuf[0];
+    a.seed = (unsigned)c->itc;
+    hipLaunchKernelGGL(thermal_init_fn(c, c->p.omega_rot != 0.0), thermal_grid(c), c->block, 0, c->stream, a, d_rho,
+                       d_u, d_v, d_w, d_T);
     return lbm_check_launch();
 }
 
+// nsteps steps.  With halo_loopback the exchange runs before every step: the
+// outgoing regions of the current state are packed and unpacked into the
+// opposite faces' incoming regions (the regions the step reads).
 int thermal_step(lbm_ctx* c, int nsteps) {
     ThermalArgs a = thermal_args(c);
     const bool rot = c->p.omega_rot != 0.0;
+    const bool halo = c->p.halo_faces != 0;
+    const bool loop = c->p.halo_loopback != 0;
     const dim3 grid = thermal_grid(c);
     for (int s = 0; s < nsteps; ++s) {
-        a.fsrc = static_cast<const double*>(c->buf[c->cur]);
-        a.fdst = static_cast<double*>(c->buf[1 - c->cur]);
-        a.gsrc = static_cast<const double*>(c->gbuf[c->cur]);
-        a.gdst = static_cast<double*>(c->gbuf[1 - c->cur]);
-        const bool macro = s == nsteps - 1;
-        void (*kern)(const ThermalArgs) = rot ? (macro ? k_thermal_step<true, true> : k_thermal_step<true, false>)
-                                              : (macro ? k_thermal_step<false, true> : k_thermal_step<false, false>);
-        hipLaunchKernelGGL(kern, grid, c->block, c->step_lds_halo, c->stream, a);
+        if (loop) {
+            int rc = halo_loopback_exchange(c);
+            if (rc) return rc;
+        }
+        a.fsrc = c->buf[c->cur];
+        a.fdst = c->buf[1 - c->cur];
+        a.gsrc = c->gbuf[c->cur];
+        a.gdst = c->gbuf[1 - c->cur];
+        a.seed = (unsigned)(c->itc + s + 1);
+        hipLaunchKernelGGL(thermal_step_fn(c, rot, s == nsteps - 1, halo), grid, c->block, c->step_lds_halo,
+                           c->stream, a);
         c->cur = 1 - c->cur;
     }
     return lbm_check_launch();

@@ -24,7 +24,7 @@ module buoyancy_data
     real(8) :: noise_amplitude = 0.0d0        ! seeded uniform noise on the initial T
     integer :: noise_seed = 2024
     integer :: itc_limit = -1                 ! >= 0: overrides itc_max
-    character(len=8) :: storage = 'F64'       ! the buoyancy path computes and stores FP64
+    character(len=8) :: storage = 'F64'       ! population storage: F64 (reference arithmetic), F32 or F16
     character(len=256) :: output_dir = '.'
     logical :: write_tecplot = .true.
     logical :: write_binary = .false.

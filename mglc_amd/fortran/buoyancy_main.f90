@@ -56,16 +56,24 @@ program buoyancy_main
         write(*,*) 'error: heated_axis must be 0, 1 or 2'
         error stop 1
     endif
-    if (storage(1:3) /= 'F64' .and. storage(1:3) /= 'f64') then
-        write(*,*) 'error: the buoyancy path stores FP64 populations (storage = ''F64'')'
-        error stop 1
-    endif
 
     p%nx = nx
     p%ny = ny
     p%nz = nz
     p%collision = LBM_MRT
-    p%storage = LBM_STORE_F64
+    ! F64: the reference's FP64 arithmetic; F32 / F16: populations stored as
+    ! FP32 deviations or scaled FP16, FP32 arithmetic, FP64 reductions
+    select case (storage(1:3))
+    case ('F32', 'f32')
+        p%storage = LBM_STORE_F32
+    case ('F16', 'f16')
+        p%storage = LBM_STORE_F16
+    case ('F64', 'f64')
+        p%storage = LBM_STORE_F64
+    case default
+        write(*,*) 'error: storage must be F64, F32 or F16'
+        error stop 1
+    end select
     p%meq12_rho = 0
     p%f16_scale_log2 = 0
     p%thermal = 1

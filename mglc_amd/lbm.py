@@ -37,7 +37,7 @@ EXPORTED_SYMBOLS = (
     "lbm_batch_autotune", "lbm_batch_set_launch", "lbm_batch_synchronize", "lbm_batch_download_macro",
     "lbm_batch_get_info", "lbm_particles_init", "lbm_particles_get", "lbm_particles_map",
     "lbm_state_bytes", "lbm_save_state", "lbm_load_state", "lbm_reynolds", "lbm_bandwidth_probe",
-    "lbm_step_begin", "lbm_step_end", "lbm_check_sums_noalloc",
+    "lbm_step_begin", "lbm_step_end", "lbm_check_sums_noalloc", "lbm_batch_set_cells_per_thread",
 )
 
 FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = range(6)
@@ -73,7 +73,7 @@ class LbmBatchInfo(ctypes.Structure):
         ("instances", ctypes.c_longlong), ("total_cells", ctypes.c_longlong), ("total_tiles", ctypes.c_longlong),
         ("steps_done", ctypes.c_longlong), ("device_bytes", ctypes.c_longlong), ("bytes_per_lup", ctypes.c_double),
         ("threads", ctypes.c_int), ("blocks", ctypes.c_int), ("graph_steps", ctypes.c_int),
-        ("stream", ctypes.c_void_p),
+        ("cells_per_thread", ctypes.c_int), ("stream", ctypes.c_void_p),
     ]
 
 
@@ -133,6 +133,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lbm_batch_step_each.argtypes = [vp, ip]
     lib.lbm_batch_autotune.argtypes = [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(ip)]
     lib.lbm_batch_set_launch.argtypes = [vp, ip, ip]
+    lib.lbm_batch_set_cells_per_thread.argtypes = [vp, ip]
     lib.lbm_batch_synchronize.argtypes = [vp]
     lib.lbm_batch_download_macro.argtypes = [vp, ip, dp, dp, dp, dp]
     lib.lbm_batch_get_info.argtypes = [vp, ctypes.POINTER(LbmBatchInfo)]
@@ -212,6 +213,7 @@ class CavityConfig:
         p.block_x, p.block_y = self.block_x, self.block_y
         p.halo_faces, p.halo_loopback, p.halo_overlap = self.halo_faces, self.halo_loopback, self.halo_overlap
         p.waves_per_cu = self.waves_per_cu
+        p.halo_faces, p.halo_loopback, p.halo_overlap = self.halo_faces, self.halo_loopback, self.halo_overlap
         p.transform = TRANSFORM_NAMES[self.transform]
         p.tau, p.u_lid, p.rho0 = self.tau(), self.U0, self.rho0
         return p
@@ -373,6 +375,11 @@ class ThermalConfig:
     N: int | None = None           # length used in the parameter statements (default nz)
     block_x: int = 0
     block_y: int = 0
+    storage: str = "f64"           # f64, or f32 / f16: FP32 arithmetic on shifted FP32 / scaled FP16 populations
+    waves_per_cu: int = 0
+    halo_faces: int = 0            # as CavityConfig: faces bordering neighbour subdomains
+    halo_loopback: int = 0
+    halo_overlap: int = 0
 
     def derived(self):
         import math
@@ -390,8 +397,11 @@ class ThermalConfig:
         d = self.derived()
         p = LbmParams()
         p.nx, p.ny, p.nz = self.nx, self.ny, self.nz
-        p.collision, p.storage, p.thermal = LBM_MRT, STORE_F64, 1
+        p.collision, p.thermal = LBM_MRT, 1
+        p.storage = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16}[self.storage]
         p.block_x, p.block_y = self.block_x, self.block_y
+        p.waves_per_cu = self.waves_per_cu
+        p.halo_faces, p.halo_loopback, p.halo_overlap = self.halo_faces, self.halo_loopback, self.halo_overlap
         p.thermal_axis = self.thermal_axis
         p.tau, p.u_lid, p.rho0 = d["tau"], 0.0, 1.0
         p.para_a, p.g_beta, p.omega_rot = d["para_a"], d["g_beta"], d["omega_rot"]
@@ -608,6 +618,9 @@ class Batch:
 
     def set_launch(self, threads: int, blocks: int):
         self._call("lbm_batch_set_launch", self._b, int(threads), int(blocks))
+
+    def set_cells_per_thread(self, cpt: int):
+        self._call("lbm_batch_set_cells_per_thread", self._b, int(cpt))
 
     def synchronize(self):
         self._call("lbm_batch_synchronize", self._b)

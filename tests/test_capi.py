@@ -36,7 +36,7 @@ def test_create_rejects_bad_tau_and_grid(hip_lib):
     p = lbm.CavityConfig(8, 8, 8, block_x=48).params()
     assert hip_lib.lbm_create(ctypes.byref(p), ctypes.byref(ctx)) == 1
     p = lbm.ThermalConfig(8, 8, 8).params()
-    p.storage = 1
+    p.collision = 1  # BGK: the thermal path's flow collision is MRT
     assert hip_lib.lbm_create(ctypes.byref(p), ctypes.byref(ctx)) == 7  # LBM_ERR_UNSUPPORTED
     p = lbm.ThermalConfig(8, 8, 8).params()
     p.u_lid = 0.1
@@ -167,3 +167,7 @@ def test_check_sums_noalloc_rejects_bad_arguments(hip_lib):
     assert hip_lib.lbm_check_sums_noalloc(p, p, p, p, p, 0, p, None) == 1
     assert hip_lib.lbm_check_sums_noalloc(p, p, p, p, p, (1 << 40) + 1, p, None) == 1
     assert b"outside [1, 2^40]" in hip_lib.lbm_last_error()
+
+
+def test_batch_cells_per_thread_rejects_bad_arguments(hip_lib):
+    assert hip_lib.lbm_batch_set_cells_per_thread(None, 2) == 1

@@ -64,6 +64,49 @@ def test_autotune_keeps_results_and_picks_a_valid_shape():
                 assert np.array_equal(g, r)
 
 
+@pytest.mark.parametrize("cpt", [2, 4])
+def test_cells_per_thread_change_nothing(cpt):
+    """Each wave taking 2 or 4 consecutive tiles per turn: the same cells, the
+    same per-cell code, the same bits as one tile per turn."""
+    cfgs = batch_configs(count=7, seed=5, sizes=(16, 24, 40))
+    with Batch(cfgs) as b1, Batch(cfgs) as b2:
+        b2.set_cells_per_thread(cpt)
+        assert b2.info().cells_per_thread == cpt
+        for b in (b1, b2):
+            b.initial()
+            b.step(230)
+        for m in range(len(cfgs)):
+            for g, r in zip(b2.macro(m), b1.macro(m)):
+                assert np.array_equal(g, r), (cpt, m)
+
+
+def test_full_batch_10000_steps_equals_single_contexts():
+    """The batched workload itself (64 cavities, N in {32..96}, FP32, autotuned
+    launch, 100-step graph replays) for its 10000 steps: one instance of every
+    size present equals a single-context run of the same parameters bit for
+    bit."""
+    cfgs = batch_configs()
+    with Batch(cfgs) as b:
+        threads, blocks = b.autotune(trial_steps=2)
+        info = b.info()
+        print(f"[gpu] full batch: {threads} threads x {blocks} blocks, {info.cells_per_thread} cells per thread",
+              flush=True)
+        b.initial()
+        for _ in range(10):
+            b.step(1000)
+        b.synchronize()
+        sample = {}
+        for m, c in enumerate(cfgs):
+            sample.setdefault(c.nx, m)
+        assert sorted(sample) == sorted({c.nx for c in cfgs})
+        got = {m: b.macro(m) for m in sample.values()}
+    for n, m in sorted(sample.items()):
+        print(f"[gpu] single {n}^3 (instance {m}) 10000 steps", flush=True)
+        ref = single_runs([cfgs[m]], 10000)[0]
+        for g, r in zip(got[m], ref):
+            assert np.array_equal(g, r), (n, m)
+
+
 def test_batch_config_sizes_and_throughput_vs_one_at_a_time():
     """The batched case (64 cavities, N in {32..96}, FP32): the persistent
     graph-replayed path against launching the instances one at a time, on the

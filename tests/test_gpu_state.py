@@ -52,8 +52,9 @@ def test_cavity_restart_with_halo_loopback_is_exact():
     restart_matches(CavityConfig(32, 30, 28, Reynolds=300, storage="f32", halo_faces=0x0F, halo_loopback=1), 50, 40)
 
 
-def test_thermal_restart_is_exact():
-    cfg = ThermalConfig(20, 24, 22, Rayleigh=1e5, Ekman=0.001)
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
+def test_thermal_restart_is_exact(storage):
+    cfg = ThermalConfig(20, 24, 22, Rayleigh=1e5, Ekman=0.001, storage=storage)
     restart_matches(cfg, 300, 200, thermal=True)
 
 
