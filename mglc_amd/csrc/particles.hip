@@ -4,21 +4,17 @@
 // Per step, all on the device (no host round trip):
 //   k_fluid_sum/mean  rhoAvg over fluid nodes of the last macro
 //                     (particle_bounceback.F90:15-35)
-//   k_link_scan       one block per sphere: the boundary links (fluid node x,
+//   k_links           one block per sphere: the boundary links (fluid node x,
 //                     direction a with x + e_a inside the sphere) of the
 //                     sphere's shell, compacted in a fixed order into the
-//                     sphere's slab of the link list
-//   k_link_prefix     waves of 64 links per sphere, prefix-summed; k_link_waves
-//                     tabulates the sphere of each wave
-//   k_link_proc       one lane per link over the whole list: wall distance q
+//                     sphere's slab; then one lane per link: wall distance q
 //                     by bisection, quadratic interpolated bounce-back with the
 //                     moving-wall term (particle_bounceback.F90:37-141), the
 //                     value stored in the solid node's own slot of the source
 //                     buffer (the slot the fluid node pulls from), and the
 //                     link's momentum exchange and torque
-//                     (particle_force.F90:31-92) summed per wave in FP64
-//   k_link_reduce     per sphere, the wave sums in order: force and torque are
-//                     deterministic
+//                     (particle_force.F90:31-92), summed in FP64 in a fixed
+//                     order: force and torque are deterministic
 //   k_particle_step   every node: plain pull streaming with halfway bounce-back
 //                     at the box walls, macro (fluid nodes), collision
 //   k_particle_update one wave per sphere: spring and buoyancy forces and the
@@ -65,32 +61,35 @@ struct ParticleState {
     int cur = 0;             // current map / sphere state
     // boundary-link list: a slab per sphere, sized for its largest shell
     unsigned* items = nullptr;
+    unsigned char* item_fluid1 = nullptr;  // per entry: the node x - e_a is in the box and fluid
     long long* slab_off = nullptr;  // np: first entry of each slab
     int* slab_cap = nullptr;        // np: entries of each slab
-    int* link_count = nullptr;      // np: links found this step
-    int* wave_off = nullptr;        // np + 1: first wave of each sphere, total
-    int* wave_sphere = nullptr;     // max_waves: the sphere of each wave
-    double* partial = nullptr;      // 6 per wave: force and torque sums
-    long long max_waves = 0;
     int* refill = nullptr;        // refill_cap pairs (node, sphere) of uncovered nodes
     int* refill_count = nullptr;  // entries this step
     int refill_cap = 0;
     double* scal = nullptr;  // [0] rhoAvg of the streaming, [1] rhoAvg of the refill
-    double* red = nullptr;   // reduction partials
-    int* err = nullptr;      // device stop record {code, step}; code 0 = running
+    double* red = nullptr;   // reduction partials (k_fluid_sum)
+    int* err = nullptr;      // device {stop code (0 = running), step of the stop, number of the step being made,
+                             // a sphere moved 1/4 node this step}
+    int* alone = nullptr;    // np flags of k_particle_update for k_particle_mark
     bool stopped = false;    // a stop was reported: no further steps
+    // graphs of GRAPH_STEPS steps without macro output, one per pair of
+    // (fluid buffer, map) parities at their start, captured on cap_stream
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace {
 
 constexpr double PI = 3.14159265358979323846;
 constexpr int REFILL_BLOCKS = 128;  // k_refill_proc: lanes walk the list with a grid stride
+constexpr int GRAPH_STEPS = 10;    // steps per graph replay; even, so the buffers end where they started
 constexpr int FLUID_BLOCKS = 4096;  // k_fluid_sum: enough waves in flight to stream the map and rho
 
 struct PArgs {
     const void* src;
     void* dst;
-    void* src_w;  // the source buffer, written by k_link_proc at solid nodes only
+    void* src_w;  // the source buffer, written by k_links at solid nodes only
     int nx, ny, nz, px;
     long long pxy, ps;
     double snu, sq;
@@ -104,17 +103,14 @@ struct PArgs {
     const double* r;
     const double* rho_avg;  // device scalars: [0] streaming, [1] refill
     double *rho, *u, *v, *w;
-    int* err;  // {code, step}: first stop wins
-    int step;  // number of the step being made
+    int* err;  // {code, step, step counter, some sphere moved 1/4 node}: first stop wins
+    int* alone;  // per sphere: no other sphere within one node (k_particle_update)
     int np;
     // boundary links
     unsigned* items;
+    unsigned char* item_fluid1;
     const long long* slab_off;
     const int* slab_cap;
-    int* link_count;
-    int* wave_off;
-    int* wave_sphere;
-    double* partial;
     // sphere update
     double gravity, threshold_wall, stiff_wall, threshold_particle, stiff_particle;
     double *F, *T, *Ftot;
@@ -149,9 +145,10 @@ __device__ __forceinline__ void body_velocity(const double* xc, const double* U,
     uz = U[3 * id + 2] + wz;
 }
 
-// Record a stop: the first code raised in a step call wins, with its step.
+// Record a stop: the first code raised in a step call wins, with its step (the
+// device step counter, advanced by the step's first kernel).
 __device__ __forceinline__ void stop(const PArgs& a, int code) {
-    if (atomicCAS(a.err, 0, code) == 0) atomicExch(a.err + 1, a.step);
+    if (atomicCAS(a.err, 0, code) == 0) atomicExch(a.err + 1, *(volatile const int*)(a.err + 2));
 }
 
 // Every particle kernel returns at once after a stop, so the state stays that
@@ -211,22 +208,154 @@ __device__ __forceinline__ double wave_sum(double x) {
     return x;
 }
 
+// Block-wide sums of x and y (RED_THREADS threads), in a fixed order; every
+// thread gets the totals.
+__device__ __forceinline__ void block_pair_sum(double& x, double& y) {
+    __shared__ double sh[2][RED_THREADS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    x = wave_sum(x);
+    y = wave_sum(y);
+    if (lane == 0) {
+        sh[0][wid] = x;
+        sh[1][wid] = y;
+    }
+    __syncthreads();
+    x = y = 0.0;
+    for (int w = 0; w < RED_THREADS / 64; ++w) {
+        x += sh[0][w];
+        y += sh[1][w];
+    }
+}
+
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_CHUNK = 2048;  // bounding-box nodes tested per block round
 
-// Links (fluid node x, direction a with x + e_a owned by the sphere) of one
-// sphere's shell r < |x - c| <= r + 1.5 (the map's own distance test), in a
-// fixed order into its slab.  The bounding box is walked in chunks of
-// SCAN_CHUNK nodes (x fastest): the distance test alone picks the chunk's shell
-// nodes, which are compacted in LDS in box order; then every lane takes one
-// shell node and loads the owner words of the node and its 18 neighbours in
-// one go, so the gathers run with full waves (about one box node in seven is
-// in the shell).  Entries are written in one pass at a block-wide running
-// offset: per round of 256 shell nodes the waves in order, inside a wave
-// direction by direction, lane by lane -- 64 consecutive entries share a
-// direction and run along x, so the link pass's gathers from the same
-// population array fall on few cache lines.
-__global__ __launch_bounds__(SCAN_THREADS) void k_link_scan(const PArgs a) {
+// A sphere's state of this step, read once per block by the link pass.
+struct SphereNow {
+    double c[3], u[3], om[3], r;
+};
+
+// One boundary link, entry `it` of the sphere `sp` whose bounding box starts at
+// lo (fluid node x, direction a with x + e_a inside the sphere), in two
+// parts, so that one link's gathers are in flight while the next link's wall
+// distance is computed:
+//   link_begin: wall distance q and boundary point x0 (calQ,
+//     particle_bounceback.F90:101-141), then the loads the interpolation
+//     needs;
+//   link_end: f(abar, x) by quadratic interpolated bounce-back with the
+//     moving-wall term (particle_bounceback.F90:37-96), stored in the source
+//     buffer at the solid node x + e_a, direction abar (exactly where the
+//     fluid node's pull along abar reads), and the momentum exchange
+//     (e_a - u_w) f_post(a, x) - (e_abar - u_w) f(abar, x) with its torque
+//     (particle_force.F90:31-92), added to acc.
+struct LinkWork {
+    long long c;
+    int al;
+    bool near, fl1, in2;
+    int o2;
+    double x0, y0, z0, qq, fa, s1, s2;
+};
+
+template <class S>
+__device__ __forceinline__ LinkWork link_begin(const PArgs& a, const SphereNow& sp, const int (&lo)[3], unsigned it,
+                                               bool fl1) {
+    using T = typename S::T;
+    const T* src = static_cast<const T*>(a.src);
+    LinkWork w;
+    const int al = (int)(it & 31u), bd = OPP[al];
+    w.al = al;
+    const int i = lo[0] + (int)(it >> 23), j = lo[1] + (int)((it >> 14) & 511u),
+              k = lo[2] + (int)((it >> 5) & 511u);
+    const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
+    w.c = c;
+    if (!calq(sp.c[0], sp.c[1], sp.c[2], sp.r, i + 1.0, j + 1.0, k + 1.0, EX[al], EY[al], EZ[al], w.x0, w.y0, w.z0,
+              w.qq))
+        stop(a, 3);
+    const bool near = w.qq < 0.5;
+    w.near = near;
+    const int i1 = i - EX[al], j1 = j - EY[al], k1 = k - EZ[al];
+    const int i2 = i1 - EX[al], j2 = j1 - EY[al], k2 = k1 - EZ[al];
+    // x - e_a: in the box and fluid is the scan's flag; x - 2 e_a is read
+    // only by the q < 1/2 branch
+    w.fl1 = fl1;
+    w.in2 = in_box(a, i2, j2, k2);
+    const int ci1 = min(max(i1, 0), a.nx - 1), cj1 = min(max(j1, 0), a.ny - 1), ck1 = min(max(k1, 0), a.nz - 1);
+    const int ci2 = min(max(i2, 0), a.nx - 1), cj2 = min(max(j2, 0), a.ny - 1), ck2 = min(max(k2, 0), a.nz - 1);
+    const long long c1 = ci1 + (long long)a.px * (cj1 + (long long)a.ny * ck1);
+    const long long c2 = ci2 + (long long)a.px * (cj2 + (long long)a.ny * ck2);
+    w.o2 = near ? a.owner[ci2 + (long long)a.nx * (cj2 + (long long)a.ny * ck2)] : 0;
+    w.fa = abs_pop<S>(src, a.ps, c, al, a.inv_scale);
+    w.s1 = abs_pop<S>(src, a.ps, c1, near ? al : bd, a.inv_scale);
+    w.s2 = abs_pop<S>(src, a.ps, near ? c2 : c, near ? al : bd, a.inv_scale);
+    return w;
+}
+
+template <class S>
+__device__ __forceinline__ void link_end(const PArgs& a, const SphereNow& sp, const LinkWork& w, double (&acc)[6]) {
+    using T = typename S::T;
+    using R = typename S::Real;
+    T* src_w = static_cast<T*>(a.src_w);
+    const int al = w.al, bd = OPP[al];
+    const double cx = sp.c[0], cy = sp.c[1], cz = sp.c[2];
+    const double x0 = w.x0, y0 = w.y0, z0 = w.z0, qq = w.qq, fa = w.fa, s1 = w.s1, s2 = w.s2;
+    const bool fl1 = w.fl1, fl2 = w.in2 && w.o2 < 0;
+    const double rx0 = x0 - cx, ry0 = y0 - cy, rz0 = z0 - cz;
+    const double wx = sp.om[1] * rz0 - sp.om[2] * ry0, wy = sp.om[2] * rx0 - sp.om[0] * rz0,
+                 wz = sp.om[0] * ry0 - sp.om[1] * rx0;  // body_spin
+    const double ux = sp.u[0], uy = sp.u[1], uz = sp.u[2];
+    const double eu = EX[bd] * (ux + wx) + EY[bd] * (uy + wy) + EZ[bd] * (uz + wz);
+    const double rho_avg = a.rho_avg[0];
+    double val;
+    if (w.near) {
+        const double f1 = fl1 ? s1 : weight(al) * a.rho0;
+        const double f2 = fl2 ? s2 : weight(al) * a.rho0;
+        val = qq * (1.0 + 2.0 * qq) * fa + (1.0 - 4.0 * qq * qq) * f1 - qq * (1.0 - 2.0 * qq) * f2 +
+              6.0 * weight(al) * rho_avg * eu;
+    } else {
+        const double fb = s2, fb1 = fl1 ? s1 : weight(bd) * a.rho0;
+        const double iq = 1.0 / qq, i12 = 1.0 / (1.0 + 2.0 * qq);
+        val = fa * iq * i12 + fb * (2.0 * qq - 1.0) * iq - fb1 * (2.0 * qq - 1.0) * i12 +
+              6.0 * weight(al) * rho_avg * iq * i12 * eu;
+    }
+    const long long cs = w.c + EX[al] + (long long)EY[al] * a.px + (long long)EZ[al] * a.pxy;
+    S::st(src_w + (long long)bd * a.ps + cs, S::SHIFT ? R(val - weight(bd)) : R(val), a.scale, 0u);
+    const double dFx = (EX[al] - ux - wx) * fa - (EX[bd] - ux - wx) * val;
+    const double dFy = (EY[al] - uy - wy) * fa - (EY[bd] - uy - wy) * val;
+    const double dFz = (EZ[al] - uz - wz) * fa - (EZ[bd] - uz - wz) * val;
+    const double rx = x0 - cx, ry = y0 - cy, rz = z0 - cz;
+    acc[0] += dFx;
+    acc[1] += dFy;
+    acc[2] += dFz;
+    acc[3] += ry * dFz - rz * dFy;
+    acc[4] += rz * dFx - rx * dFz;
+    acc[5] += rx * dFy - ry * dFx;
+}
+
+// The boundary links of one sphere and what they exchange, one block per
+// sphere (all of a sphere's gathers go through one CU's L1):
+//  1. the links (fluid node x, direction a with x + e_a owned by the sphere)
+//     of the shell r < |x - c| <= r + 1.5 (the map's own distance test), in a
+//     fixed order into the sphere's slab, each with a flag: x - e_a is in the
+//     box and fluid (the scan has loaded that neighbour's owner word).  The bounding box is walked in
+//     chunks of SCAN_CHUNK nodes (x fastest): the distance test alone picks
+//     the chunk's shell nodes, which are compacted in LDS in box order; then
+//     every lane takes one shell node and loads the owner words of the node
+//     and its 18 neighbours in one go, so the gathers run with full waves
+//     (about one box node in seven is in the shell).  Entries are written in
+//     one pass at a block-wide running offset: per round of 256 shell nodes
+//     the waves in order, inside a wave direction by direction, lane by lane;
+//  2. one lane per link, two at a time (entries e and e + 256, e = 64 w +
+//     512 t + lane for wave w): link_begin of both, then link_end of both;
+//  3. force and torque: per lane in entry order, per wave by a fixed shuffle
+//     tree, then the four waves in order -- the same sums on every run.
+// Measured on MI355X at 192^3 with 2000 spheres: ordering the entries
+// direction by direction over the whole sphere, or giving each XCD the
+// spheres of one region of the box, fetched less or the same and ran no
+// faster, and neither did more or fewer registers (occupancy); the link pass
+// is bound by its scattered gathers, so what counts is fewer of them and
+// more of them in flight.
+template <class S>
+__global__ __launch_bounds__(SCAN_THREADS) void k_links(const PArgs a) {
     __shared__ int halt;  // read once, so the whole block leaves together (barriers follow)
     __shared__ unsigned shell[SCAN_CHUNK];
     __shared__ int part[SCAN_CHUNK / SCAN_THREADS][SCAN_THREADS / 64];
@@ -242,6 +371,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_link_scan(const PArgs a) {
     const double cx = a.xc[3 * id], cy = a.xc[3 * id + 1], cz = a.xc[3 * id + 2], rad = a.r[id];
     const double rin2 = rad * rad, rout2 = (rad + 1.5) * (rad + 1.5);
     unsigned* list = a.items + a.slab_off[id];
+    unsigned char* fluid1 = a.item_fluid1 + a.slab_off[id];
     const int cap = a.slab_cap[id];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     constexpr int PER = SCAN_CHUNK / SCAN_THREADS;
@@ -289,7 +419,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_link_scan(const PArgs a) {
         // 2. one lane per shell node: its links, then the entries
         for (int r0 = 0; r0 < nshell; r0 += SCAN_THREADS) {
             const int q = r0 + (int)threadIdx.x;
-            unsigned mk = 0u, e0 = 0u;
+            unsigned mk = 0u, e0 = 0u, fl1mask = 0u;
             if (q < nshell) {
                 int i, j, k;
                 coords(shell[q], i, j, k);
@@ -306,6 +436,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_link_scan(const PArgs a) {
                     sfor<Q - 1>([&](auto B) {
                         constexpr int al = decltype(B)::value + 1;
                         if (in_box(a, i + EX[al], j + EY[al], k + EZ[al]) && nb[al] == id) mk |= 1u << al;
+                        // the link's first stencil point x - e_a (the neighbour
+                        // along the opposite direction): in the box and fluid
+                        if (in_box(a, i - EX[al], j - EY[al], k - EZ[al]) && nb[OPP[al]] < 0) fl1mask |= 1u << al;
                     });
                 }
             }
@@ -324,174 +457,59 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_link_scan(const PArgs a) {
                 const unsigned long long b = __ballot(has);
                 const int before = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32),
                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
-                if (has && at + before < cap) list[at + before] = e0 | (unsigned)al;
+                if (has && at + before < cap) {
+                    list[at + before] = e0 | (unsigned)al;
+                    fluid1[at + before] = (unsigned char)((fl1mask >> al) & 1u);
+                }
                 at += (int)__popcll(b);
             });
             pos += all;
             __syncthreads();  // wave_tot is rewritten by the next round
         }
     }
-    if (threadIdx.x == 0) {
-        if (pos > cap) {  // cannot happen for a slab sized by the shell bound
-            stop(a, 5);
-            a.link_count[id] = 0;
-        } else {
-            a.link_count[id] = pos;
-        }
+    // pos: the sphere's entries (block-uniform)
+    if (pos > cap) {  // cannot happen for a slab sized by the shell bound
+        if (threadIdx.x == 0) stop(a, 5);
+        return;
     }
-}
-
-// wave_off[id] = waves (64 links each) of the spheres before id; wave_off[np]
-// = all waves.  One block.
-__global__ __launch_bounds__(1024) void k_link_prefix(const PArgs a) {
-    __shared__ int wave_tot[16];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    int carry = 0;
-    for (int base = 0; base < a.np; base += 1024) {
-        const int id = base + threadIdx.x;
-        const int v = id < a.np ? (a.link_count[id] + 63) / 64 : 0;
-        int incl = v;
-        for (int off = 1; off < 64; off <<= 1) {
-            const int y = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += y;
-        }
-        if (lane == 63) wave_tot[wid] = incl;
-        __syncthreads();
-        int before = carry, all = 0;
-        for (int w = 0; w < 16; ++w) {
-            if (w < wid) before += wave_tot[w];
-            all += wave_tot[w];
-        }
-        if (id < a.np) a.wave_off[id] = before + incl - v;
-        carry += all;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) a.wave_off[a.np] = carry;
-}
-
-// wave_sphere[w] = the sphere whose links wave w carries.
-__global__ __launch_bounds__(64) void k_link_waves(const PArgs a) {
-    const int id = blockIdx.x;
-    for (int w = a.wave_off[id] + threadIdx.x; w < a.wave_off[id + 1]; w += 64) a.wave_sphere[w] = id;
-}
-
-// One lane per link, 64 links of one sphere per wave, over all spheres:
-//   - wall distance q and boundary point x0 (calQ, particle_bounceback.F90:101-141)
-//   - f(abar, x) by quadratic interpolated bounce-back with the moving-wall
-//     term (particle_bounceback.F90:37-96), stored in the source buffer at the
-//     solid node x + e_a, direction abar: exactly where the fluid node's pull
-//     along abar reads
-//   - the momentum exchange (e_a - u_w) f_post(a, x) - (e_abar - u_w) f(abar, x)
-//     and its torque (particle_force.F90:31-92), summed over the wave in FP64
-//     (fixed shuffle tree) into the wave's partial.
-template <class S>
-__global__ __launch_bounds__(256) void k_link_proc(const PArgs a) {
-    using T = typename S::T;
-    using R = typename S::Real;
-    if (__builtin_amdgcn_readfirstlane(stopped(a) ? 1 : 0)) return;
-    const int lane = threadIdx.x & 63;
-    const T* src = static_cast<const T*>(a.src);
-    T* src_w = static_cast<T*>(a.src_w);
-    const int total = a.wave_off[a.np];
-    const int stride = gridDim.x * (blockDim.x / 64);
-    for (int w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); w < total; w += stride) {
-        const int id = __builtin_amdgcn_readfirstlane(a.wave_sphere[w]);
-        const int idx = (w - a.wave_off[id]) * 64 + lane;
-        double acc[6] = {0, 0, 0, 0, 0, 0};
-        if (idx < a.link_count[id]) {
-            int lo[3], hi[3];
-            sphere_box(a, a.xc, id, 1, lo, hi);
-            const unsigned it = a.items[a.slab_off[id] + idx];
-            const int al = (int)(it & 31u), bd = OPP[al];
-            const int i = lo[0] + (int)(it >> 23), j = lo[1] + (int)((it >> 14) & 511u),
-                      k = lo[2] + (int)((it >> 5) & 511u);
-            const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
-            // the wall distance first: it decides which three populations the
-            // interpolation reads, so each link gathers three values from the
-            // source buffer (the pass is bound by these scattered reads)
-            const double cx = a.xc[3 * id], cy = a.xc[3 * id + 1], cz = a.xc[3 * id + 2];
-            double x0, y0, z0, qq;
-            if (!calq(cx, cy, cz, a.r[id], i + 1.0, j + 1.0, k + 1.0, EX[al], EY[al], EZ[al], x0, y0, z0, qq))
-                stop(a, 3);
-            const bool near = qq < 0.5;
-            // stencil points x - e_a and x - 2 e_a, clamped into the box; a
-            // point outside it or in a solid takes the rest equilibrium w rho0
-            const int i1 = i - EX[al], j1 = j - EY[al], k1 = k - EZ[al];
-            const int i2 = i1 - EX[al], j2 = j1 - EY[al], k2 = k1 - EZ[al];
-            const bool in1 = in_box(a, i1, j1, k1), in2 = in_box(a, i2, j2, k2);
-            const int ci1 = min(max(i1, 0), a.nx - 1), cj1 = min(max(j1, 0), a.ny - 1), ck1 = min(max(k1, 0), a.nz - 1);
-            const int ci2 = min(max(i2, 0), a.nx - 1), cj2 = min(max(j2, 0), a.ny - 1), ck2 = min(max(k2, 0), a.nz - 1);
-            const long long c1 = ci1 + (long long)a.px * (cj1 + (long long)a.ny * ck1);
-            const long long c2 = ci2 + (long long)a.px * (cj2 + (long long)a.ny * ck2);
-            const int o1 = a.owner[ci1 + (long long)a.nx * (cj1 + (long long)a.ny * ck1)];
-            const int o2 = a.owner[ci2 + (long long)a.nx * (cj2 + (long long)a.ny * ck2)];
-            // q < 1/2: f_post(a) at x, x - e_a, x - 2 e_a; q >= 1/2: f_post(a)
-            // and f_post(abar) at x, f_post(abar) at x - e_a
-            const double fa = abs_pop<S>(src, a.ps, c, al, a.inv_scale);
-            const double s1 = abs_pop<S>(src, a.ps, c1, near ? al : bd, a.inv_scale);
-            const double s2 = abs_pop<S>(src, a.ps, near ? c2 : c, near ? al : bd, a.inv_scale);
-            const bool fl1 = in1 && o1 < 0, fl2 = in2 && o2 < 0;
-            double wx, wy, wz;
-            body_spin(a.xc, a.Om, id, x0, y0, z0, wx, wy, wz);
-            const double ux = a.U[3 * id], uy = a.U[3 * id + 1], uz = a.U[3 * id + 2];
-            const double eu = EX[bd] * (ux + wx) + EY[bd] * (uy + wy) + EZ[bd] * (uz + wz);
-            const double rho_avg = a.rho_avg[0];
-            double val;
-            if (near) {
-                const double f1 = fl1 ? s1 : weight(al) * a.rho0;
-                const double f2 = fl2 ? s2 : weight(al) * a.rho0;
-                val = qq * (1.0 + 2.0 * qq) * fa + (1.0 - 4.0 * qq * qq) * f1 - qq * (1.0 - 2.0 * qq) * f2 +
-                      6.0 * weight(al) * rho_avg * eu;
-            } else {
-                const double fb = s2, fb1 = fl1 ? s1 : weight(bd) * a.rho0;
-                const double iq = 1.0 / qq, i12 = 1.0 / (1.0 + 2.0 * qq);
-                val = fa * iq * i12 + fb * (2.0 * qq - 1.0) * iq - fb1 * (2.0 * qq - 1.0) * i12 +
-                      6.0 * weight(al) * rho_avg * iq * i12 * eu;
-            }
-            const long long cs = c + EX[al] + (long long)EY[al] * a.px + (long long)EZ[al] * a.pxy;
-            S::st(src_w + (long long)bd * a.ps + cs, S::SHIFT ? R(val - weight(bd)) : R(val), a.scale, 0u);
-            // Wen et al. (JCP 2014) exchange, (e - U - W) grouped as the
-            // reference groups it (CaseIV_DKT.F90:712-713)
-            const double dFx = (EX[al] - ux - wx) * fa - (EX[bd] - ux - wx) * val;
-            const double dFy = (EY[al] - uy - wy) * fa - (EY[bd] - uy - wy) * val;
-            const double dFz = (EZ[al] - uz - wz) * fa - (EZ[bd] - uz - wz) * val;
-            const double rx = x0 - cx, ry = y0 - cy, rz = z0 - cz;
-            acc[0] = dFx;
-            acc[1] = dFy;
-            acc[2] = dFz;
-            acc[3] = ry * dFz - rz * dFy;
-            acc[4] = rz * dFx - rx * dFz;
-            acc[5] = rx * dFy - ry * dFx;
-        }
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const double v = wave_sum(acc[q]);
-            if (lane == 0) a.partial[6 * (long long)w + q] = v;
-        }
-    }
-}
-
-// Force and torque of each sphere: its wave partials summed in order.
-__global__ __launch_bounds__(64) void k_link_reduce(const PArgs a) {
-    if (__builtin_amdgcn_readfirstlane(stopped(a) ? 1 : 0)) return;
-    const int id = blockIdx.x, lane = threadIdx.x;
+    // the entries written above are visible to every wave of the block (the
+    // last barrier of step 1)
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int w = a.wave_off[id] + lane; w < a.wave_off[id + 1]; w += 64)
+    SphereNow sp;
+    for (int d = 0; d < 3; ++d) {
+        sp.c[d] = a.xc[3 * id + d];
+        sp.u[d] = a.U[3 * id + d];
+        sp.om[d] = a.Om[3 * id + d];
+    }
+    sp.r = rad;
+    for (int e = threadIdx.x; e < pos; e += 2 * SCAN_THREADS) {
+        const int e1 = e + SCAN_THREADS;
+        const int ee = min(e1, pos - 1);  // past the end: a copy of the last link, computed and dropped
+        const unsigned it0 = list[e], it1 = list[ee];
+        const bool f0 = fluid1[e] != 0, f1 = fluid1[ee] != 0;
+        const LinkWork w0 = link_begin<S>(a, sp, lo, it0, f0);
+        const LinkWork w1 = link_begin<S>(a, sp, lo, it1, f1);
+        link_end<S>(a, sp, w0, acc);
+        if (e1 < pos) link_end<S>(a, sp, w1, acc);
+    }
+    __shared__ double sums[6][SCAN_THREADS / 64];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) acc[q] += a.partial[6 * (long long)w + q];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) acc[q] = wave_sum(acc[q]);
-    if (lane == 0) {
-        for (int q = 0; q < 3; ++q) {
-            a.F[3 * id + q] = acc[q];
-            a.T[3 * id + q] = acc[3 + q];
-        }
+    for (int q = 0; q < 6; ++q) {
+        const double v = wave_sum(acc[q]);
+        if (lane == 0) sums[q][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double v = sums[threadIdx.x][0];
+        for (int w = 1; w < SCAN_THREADS / 64; ++w) v += sums[threadIdx.x][w];
+        if (threadIdx.x < 3) a.F[3 * id + threadIdx.x] = v;
+        else a.T[3 * id + threadIdx.x - 3] = v;
     }
 }
 
 // Every node: pull streaming (fluid.F90:75-161) with halfway bounce-back at
 // the box walls -- a pull out of a solid node reads the interpolated
-// bounce-back value k_link_proc stored there -- then macro (fluid.F90:164-184;
+// bounce-back value k_links stored there -- then macro (fluid.F90:164-184;
 // rho every step for rhoAvg, u, v, w on the last step of a call) on fluid
 // nodes, and the next collision (fluid.F90:1-72).  Solid nodes run the same
 // arithmetic on values nobody reads.
@@ -539,14 +557,20 @@ __global__ __launch_bounds__(256) void k_particle_step(const PArgs a) {
     });
 }
 
-// Streamed population bd of fluid node (i,j,k), as k_particle_step pulls it.
-template <class S>
-__device__ double streamed_pop(const PArgs& a, const typename S::T* src, int i, int j, int k, int bd) {
+// Streamed population al of node (i, j, k) (in the box), as k_particle_step
+// pulls it: from the neighbour upstream, or bounced back at the box wall.
+// One load at a selected address (the weights of al and its opposite agree).
+template <class S, int al>
+__device__ __forceinline__ double streamed_pop_at(const PArgs& a, const typename S::T* src, int i, int j, int k) {
     const long long c = i + (long long)a.px * (j + (long long)a.ny * k);
-    if (bd == 0) return abs_pop<S>(src, a.ps, c, 0, a.inv_scale);
-    const int si = i - EX[bd], sj = j - EY[bd], sk = k - EZ[bd];
-    if (!in_box(a, si, sj, sk)) return abs_pop<S>(src, a.ps, c, OPP[bd], a.inv_scale);
-    return abs_pop<S>(src, a.ps, si + (long long)a.px * (sj + (long long)a.ny * sk), bd, a.inv_scale);
+    long long idx = c;
+    if constexpr (al != 0) {
+        const int si = i - EX[al], sj = j - EY[al], sk = k - EZ[al];
+        idx = in_box(a, si, sj, sk) ? (long long)al * a.ps + si + (long long)a.px * (sj + (long long)a.ny * sk)
+                                    : (long long)OPP[al] * a.ps + c;
+    }
+    const double g = double(S::ld(src + idx, a.inv_scale));
+    return S::SHIFT ? g + weight(al) : g;
 }
 
 // One wave per sphere: the lanes split the pair loop of the particle springs,
@@ -565,10 +589,13 @@ __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
     double fs[3] = {0.0, 0.0, 0.0};
     const double scale_p = V * (a.rho_solid - rho_avg) * a.gravity / a.stiff_particle;
     const double px = a.xc[3 * p], py = a.xc[3 * p + 1], pz = a.xc[3 * p + 2];
+    bool close = false;  // another sphere within one node of this one
     for (int q = lane; q < a.np; q += 64) {
         if (q == p) continue;
         const double dx = px - a.xc[3 * q], dy = py - a.xc[3 * q + 1], dz = pz - a.xc[3 * q + 2];
         const double rs = rp + a.r[q];
+        const double r1 = rs + 1.0;
+        if (fabs(dx) < r1 && fabs(dy) < r1 && fabs(dz) < r1 && dist2(dx, dy, dz) < r1 * r1) close = true;
         const double reach = rs + a.threshold_particle;
         if (fabs(dx) >= reach || fabs(dy) >= reach || fabs(dz) >= reach) continue;  // dij >= reach too
         const double dij = sqrt(dist2(dx, dy, dz));
@@ -584,6 +611,7 @@ __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
         fs[2] += mag * dz / dij;
     }
     for (int d = 0; d < 3; ++d) fs[d] = wave_sum(fs[d]);
+    const bool alone = __ballot(close) == 0ull;
     if (lane != 0) return;
     // wall springs summed apart from the particle springs and added last, as
     // the reference keeps Fwxij apart from Fxij (particle_force.F90:192-193)
@@ -606,21 +634,32 @@ __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
     double Fn[3] = {a.F[3 * p] + fs[0] + fw[0], a.F[3 * p + 1] + fs[1] + fw[1],
                     a.F[3 * p + 2] - (a.rho_solid - rho_avg) * V * a.gravity + fs[2] + fw[2]};
     const double mass = a.rho_solid * V, inertia = 0.4 * mass * rp * rp;
+    double moved = 0.0;
     for (int d = 0; d < 3; ++d) {
         const double acc = Fn[d] / mass;
         const double Uold = a.U[3 * p + d];
         a.U_out[3 * p + d] = Uold + acc;
-        a.xc_out[3 * p + d] = a.xc[3 * p + d] + Uold + 0.5 * acc;
+        const double x = a.xc[3 * p + d] + Uold + 0.5 * acc;
+        a.xc_out[3 * p + d] = x;
         a.Om_out[3 * p + d] = a.Om[3 * p + d] + a.T[3 * p + d] / inertia;
         a.Ftot[3 * p + d] = Fn[d];
+        moved += (x - a.xc[3 * p + d]) * (x - a.xc[3 * p + d]);
     }
+    // k_particle_mark may claim this sphere's new nodes with plain stores if
+    // no other sphere was within one node of it and no sphere moves 1/4 node
+    // or more (or is not finite): the new balls are then at least half a node
+    // apart and share no node
+    a.alone[p] = alone ? 1 : 0;
+    if (!(moved < 0.0625)) atomicOr(a.err + 3, 1);
 }
 
-// New map from the updated spheres; solids at rest with rho_s.
-// Nodes of sphere id (d^2 <= r^2, initial.F90:78-89) walked column by column
-// of its bounding box; f(i, j, k, m) is called for each.
-template <class F>
-__device__ __forceinline__ void for_sphere_nodes(const PArgs& a, const double* xc, int id, F&& f) {
+// Nodes of sphere id (d^2 <= r^2, initial.F90:78-89), walked column by
+// column of its bounding box, four rows of a column at a time: load(m) for
+// the four nodes first, so their loads are in flight together, then
+// f(i, j, k, m, loaded value) for each.
+template <class L, class F>
+__device__ __forceinline__ void for_sphere_nodes(const PArgs& a, const double* xc, int id, L&& load, F&& f) {
+    using V = decltype(load(0LL));
     int lo[3], hi[3];
     sphere_box(a, xc, id, 1, lo, hi);
     const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1;
@@ -630,9 +669,21 @@ __device__ __forceinline__ void for_sphere_nodes(const PArgs& a, const double* x
         const int i = lo[0] + col % bx, j = lo[1] + col / bx;
         int k0, k1;
         column_rows(cx, cy, cz, a.r[id], i, j, lo[2], hi[2], k0, k1);
-        for (int k = k0; k <= k1; ++k)
-            if (dist2(i + 1.0 - cx, j + 1.0 - cy, k + 1.0 - cz) <= rr)
-                f(i, j, k, i + (long long)a.nx * (j + (long long)a.ny * k));
+        for (int kb = k0; kb <= k1; kb += 4) {
+            bool in[4];
+            long long m[4];
+            V v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = kb + u;
+                in[u] = k <= k1 && dist2(i + 1.0 - cx, j + 1.0 - cy, k + 1.0 - cz) <= rr;
+                m[u] = i + (long long)a.nx * (j + (long long)a.ny * k);
+                if (in[u]) v[u] = load(m[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (in[u]) f(i, j, kb + u, m[u], v[u]);
+        }
     }
 }
 
@@ -640,21 +691,28 @@ __device__ __forceinline__ void for_sphere_nodes(const PArgs& a, const double* x
 __global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner_out) {
     if (stopped(a)) return;
     const int id = blockIdx.x;
-    for_sphere_nodes(a, a.xc_new, id, [&](int, int, int, long long m) {
-        atomicMax(owner_out + m, id);  // a node claimed twice goes to the higher index
-        if (a.owner[m] < 0) {  // newly covered; nodes solid before already hold rho_s, u = 0
-            a.rho[m] = a.rho_solid;
-            a.u[m] = 0.0;
-            a.v[m] = 0.0;
-            a.w[m] = 0.0;
-        }
-    });
+    // a node claimed twice goes to the higher index (atomicMax); a sphere that
+    // shares no node with another (k_particle_update's test) stores plainly
+    const bool plain = a.alone[id] != 0 && *(volatile const int*)(a.err + 3) == 0;
+    for_sphere_nodes(
+        a, a.xc_new, id, [&](long long m) { return a.owner[m]; },
+        [&](int, int, int, long long m, int own) {
+            if (plain) owner_out[m] = id;
+            else atomicMax(owner_out + m, id);
+            if (own < 0) {  // newly covered; nodes solid before already hold rho_s, u = 0
+                a.rho[m] = a.rho_solid;
+                a.u[m] = 0.0;
+                a.v[m] = 0.0;
+                a.w[m] = 0.0;
+            }
+        });
 }
 
 // Initial map (from the current state).
 __global__ __launch_bounds__(256) void k_particle_init_mark(const PArgs a, int* owner_out) {
     const int id = blockIdx.x;
-    for_sphere_nodes(a, a.xc, id, [&](int, int, int, long long m) { atomicMax(owner_out + m, id); });
+    for_sphere_nodes(
+        a, a.xc, id, [](long long) { return 0; }, [&](int, int, int, long long m, int) { atomicMax(owner_out + m, id); });
 }
 
 // rho = rho_s in solids and rho0 in the fluid, u = 0, and the first
@@ -718,19 +776,38 @@ __device__ __forceinline__ void refill_node(const PArgs& a, int i, int j, int k,
         stop(a, 4);
         return;
     }
+    // the stencil points x + s e_c, s = 1, 2, 3: the streamed populations of
+    // those fluid in both maps, the rest equilibrium w rho0 for the others.
+    // A refill is one lane's chain of gathers, so every load is issued (at a
+    // point clamped into the box) before any is used.
+    int pi[3], pj[3], pk[3];
+    bool in[3];
+    int own[3][2];
+    for (int s = 0; s < 3; ++s) {
+        pi[s] = i + (s + 1) * EX[ec];
+        pj[s] = j + (s + 1) * EY[ec];
+        pk[s] = k + (s + 1) * EZ[ec];
+        in[s] = in_box(a, pi[s], pj[s], pk[s]);
+        pi[s] = min(max(pi[s], 0), a.nx - 1);
+        pj[s] = min(max(pj[s], 0), a.ny - 1);
+        pk[s] = min(max(pk[s], 0), a.nz - 1);
+        const long long pm = pi[s] + (long long)a.nx * (pj[s] + (long long)a.ny * pk[s]);
+        own[s][0] = a.owner[pm];
+        own[s][1] = a.owner_new[pm];
+    }
+    double v[3][Q];
+    for (int s = 0; s < 3; ++s)
+        sfor<Q>([&](auto B) {
+            constexpr int al = decltype(B)::value;
+            v[s][al] = streamed_pop_at<S, al>(a, src, pi[s], pj[s], pk[s]);
+        });
     double fr[Q];
-    for (int al = 0; al < Q; ++al) fr[al] = 0.0;
-    for (int s = 1; s <= 3; ++s) {
-        const int pi = i + s * EX[ec], pj = j + s * EY[ec], pk = k + s * EZ[ec];
-        bool ok = in_box(a, pi, pj, pk);
-        if (ok) {
-            const long long pm = pi + (long long)a.nx * (pj + (long long)a.ny * pk);
-            ok = a.owner[pm] < 0 && a.owner_new[pm] < 0;
-        }
-        const double coef = s == 1 ? 3.0 : (s == 2 ? -3.0 : 1.0);
+    for (int s = 0; s < 3; ++s) {
+        const bool ok = in[s] && own[s][0] < 0 && own[s][1] < 0;
+        const double coef = s == 0 ? 3.0 : (s == 1 ? -3.0 : 1.0);
         for (int al = 0; al < Q; ++al) {
-            const double v = ok ? streamed_pop<S>(a, src, pi, pj, pk, al) : weight(al) * a.rho0;
-            fr[al] = s == 1 ? coef * v : fr[al] + coef * v;
+            const double x = ok ? v[s][al] : weight(al) * a.rho0;
+            fr[al] = s == 0 ? coef * x : fr[al] + coef * x;
         }
     }
     // momentum moments replaced by rhoAvg times the body velocity
@@ -761,16 +838,18 @@ __device__ __forceinline__ void refill_node(const PArgs& a, int i, int j, int k,
 __global__ __launch_bounds__(256) void k_refill_scan(const PArgs a) {
     if (stopped(a)) return;
     const int id = blockIdx.x;
-    for_sphere_nodes(a, a.xc, id, [&](int, int, int, long long m) {
-        if (!(a.owner[m] == id && a.owner_new[m] < 0)) return;
-        const int slot = atomicAdd(a.refill_count, 1);
-        if (slot < a.refill_cap) {
-            a.refill[2 * slot] = (int)m;
-            a.refill[2 * slot + 1] = id;
-        } else {
-            stop(a, 6);
-        }
-    });
+    for_sphere_nodes(
+        a, a.xc, id, [&](long long m) { return int2{a.owner[m], a.owner_new[m]}; },
+        [&](int, int, int, long long m, int2 own) {
+            if (!(own.x == id && own.y < 0)) return;
+            const int slot = atomicAdd(a.refill_count, 1);
+            if (slot < a.refill_cap) {
+                a.refill[2 * slot] = (int)m;
+                a.refill[2 * slot + 1] = id;
+            } else {
+                stop(a, 6);
+            }
+        });
 }
 
 template <class S, int COLL>
@@ -785,7 +864,7 @@ __global__ __launch_bounds__(256) void k_refill_proc(const PArgs a) {
     }
 }
 
-// Sum of rho and count over fluid nodes, then rhoAvg into out.
+// Sum of rho and count over fluid nodes, per block.
 __global__ __launch_bounds__(RED_THREADS) void k_fluid_sum(const double* __restrict__ rho, const int* __restrict__ owner,
                                                            long long n, double* __restrict__ partial) {
     // four nodes per thread and pass: one 16-byte owner load, two 16-byte
@@ -809,49 +888,30 @@ __global__ __launch_bounds__(RED_THREADS) void k_fluid_sum(const double* __restr
         const long long c = 4 * groups + threadIdx.x;
         if (owner[c] < 0) { s += rho[c]; cnt += 1.0; }
     }
-    __shared__ double sh[2][RED_THREADS / 64];
-    s = wave_sum(s);
-    cnt = wave_sum(cnt);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) {
-        sh[0][wid] = s;
-        sh[1][wid] = cnt;
-    }
-    __syncthreads();
+    block_pair_sum(s, cnt);
     if (threadIdx.x == 0) {
-        double a = 0.0, b = 0.0;
-        for (int w = 0; w < RED_THREADS / 64; ++w) {
-            a += sh[0][w];
-            b += sh[1][w];
-        }
-        partial[2 * blockIdx.x] = a;
-        partial[2 * blockIdx.x + 1] = b;
+        partial[2 * blockIdx.x] = s;
+        partial[2 * blockIdx.x + 1] = cnt;
     }
 }
 
+// rhoAvg = the block partials summed in block order.  Given step_ctr (the
+// first launch of a step), the device step counter moves on and the step's
+// "a sphere moved 1/4 node" flag is cleared.  One block.
 __global__ __launch_bounds__(RED_THREADS) void k_fluid_mean(const double* __restrict__ partial, int nblocks,
-                                                            double* __restrict__ out) {
+                                                            double* __restrict__ out, int* step_ctr, int* moved) {
     double s = 0.0, cnt = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += RED_THREADS) {
         s += partial[2 * b];
         cnt += partial[2 * b + 1];
     }
-    __shared__ double sh[2][RED_THREADS / 64];
-    s = wave_sum(s);
-    cnt = wave_sum(cnt);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) {
-        sh[0][wid] = s;
-        sh[1][wid] = cnt;
-    }
-    __syncthreads();
+    block_pair_sum(s, cnt);
     if (threadIdx.x == 0) {
-        double a = 0.0, b = 0.0;
-        for (int w = 0; w < RED_THREADS / 64; ++w) {
-            a += sh[0][w];
-            b += sh[1][w];
+        *out = s / cnt;
+        if (step_ctr) {
+            ++*step_ctr;
+            *moved = 0;
         }
-        *out = a / b;
     }
 }
 
@@ -891,6 +951,7 @@ PArgs particle_args(const lbm_ctx* c) {
     a.v = c->macro + 2 * c->n;
     a.w = c->macro + 3 * c->n;
     a.err = s->err;
+    a.alone = s->alone;
     a.np = s->np;
     a.gravity = s->pp.gravity;
     a.threshold_wall = s->pp.threshold_wall;
@@ -901,12 +962,9 @@ PArgs particle_args(const lbm_ctx* c) {
     a.T = s->T;
     a.Ftot = s->Ftot;
     a.items = s->items;
+    a.item_fluid1 = s->item_fluid1;
     a.slab_off = s->slab_off;
     a.slab_cap = s->slab_cap;
-    a.link_count = s->link_count;
-    a.wave_off = s->wave_off;
-    a.wave_sphere = s->wave_sphere;
-    a.partial = s->partial;
     a.refill = s->refill;
     a.refill_count = s->refill_count;
     a.refill_cap = s->refill_cap;
@@ -917,52 +975,88 @@ dim3 cell_grid(const lbm_ctx* c) {
     return dim3((c->p.nx + c->block.x - 1) / c->block.x, (c->p.ny + c->block.y - 1) / c->block.y, c->p.nz);
 }
 
-void launch_fluid_mean(lbm_ctx* c, const int* owner, double* out) {
+void launch_fluid_mean(lbm_ctx* c, hipStream_t st, const int* owner, double* out, int* step_ctr) {
     ParticleState* s = c->part;
-    hipLaunchKernelGGL(k_fluid_sum, dim3(FLUID_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->macro, owner, c->n, s->red);
-    hipLaunchKernelGGL(k_fluid_mean, dim3(1), dim3(RED_THREADS), 0, c->stream, s->red, FLUID_BLOCKS, out);
+    hipLaunchKernelGGL(k_fluid_sum, dim3(FLUID_BLOCKS), dim3(RED_THREADS), 0, st, c->macro, owner, c->n, s->red);
+    hipLaunchKernelGGL(k_fluid_mean, dim3(1), dim3(RED_THREADS), 0, st, s->red, FLUID_BLOCKS, out, step_ctr,
+                       s->err + 3);
 }
 
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
+// The launches of one step on stream st; both parities flip.
 template <class S, int COLL>
-void particle_launches(lbm_ctx* c, int step, bool macro) {
+void particle_launches(lbm_ctx* c, hipStream_t st, bool macro) {
     ParticleState* s = c->part;
     const int np = s->np;
-    launch_fluid_mean(c, s->owner[s->cur], s->scal);
+    launch_fluid_mean(c, st, s->owner[s->cur], s->scal, s->err + 2);
     PArgs a = particle_args(c);
-    a.step = step;
     a.src = c->buf[c->cur];
     a.src_w = c->buf[c->cur];
     a.dst = c->buf[1 - c->cur];
-    const int proc_blocks = (int)std::min<long long>(2048, (s->max_waves + 3) / 4);
-    hipLaunchKernelGGL(k_link_scan, dim3(np), dim3(SCAN_THREADS), 0, c->stream, a);
-    hipLaunchKernelGGL(k_link_prefix, dim3(1), dim3(1024), 0, c->stream, a);
-    hipLaunchKernelGGL(k_link_waves, dim3(np), dim3(64), 0, c->stream, a);
-    hipLaunchKernelGGL(k_link_proc<S>, dim3(proc_blocks), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL(k_link_reduce, dim3(np), dim3(64), 0, c->stream, a);
+    hipLaunchKernelGGL(k_links<S>, dim3(np), dim3(SCAN_THREADS), 0, st, a);
     if (macro)
-        hipLaunchKernelGGL((k_particle_step<S, COLL, true>), cell_grid(c), c->block, 0, c->stream, a);
+        hipLaunchKernelGGL((k_particle_step<S, COLL, true>), cell_grid(c), c->block, 0, st, a);
     else
-        hipLaunchKernelGGL((k_particle_step<S, COLL, false>), cell_grid(c), c->block, 0, c->stream, a);
-    hipLaunchKernelGGL(k_particle_update, dim3((np + 3) / 4), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL(k_fill_int, dim3(2048), dim3(256), 0, c->stream, s->owner[1 - s->cur], c->n, -1);
-    hipLaunchKernelGGL(k_particle_mark, dim3(np), dim3(256), 0, c->stream, a, s->owner[1 - s->cur]);
-    launch_fluid_mean(c, s->owner[1 - s->cur], s->scal + 1);
-    hipLaunchKernelGGL(k_refill_scan, dim3(np), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL((k_refill_proc<S, COLL>), dim3(REFILL_BLOCKS), dim3(256), 0, c->stream, a);
+        hipLaunchKernelGGL((k_particle_step<S, COLL, false>), cell_grid(c), c->block, 0, st, a);
+    hipLaunchKernelGGL(k_particle_update, dim3((np + 3) / 4), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_fill_int, dim3(2048), dim3(256), 0, st, s->owner[1 - s->cur], c->n, -1);
+    hipLaunchKernelGGL(k_particle_mark, dim3(np), dim3(256), 0, st, a, s->owner[1 - s->cur]);
+    launch_fluid_mean(c, st, s->owner[1 - s->cur], s->scal + 1, nullptr);
+    hipLaunchKernelGGL(k_refill_scan, dim3(np), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_refill_proc<S, COLL>), dim3(REFILL_BLOCKS), dim3(256), 0, st, a);
     s->cur = 1 - s->cur;
     c->cur = 1 - c->cur;
 }
 
+void launch_step(lbm_ctx* c, hipStream_t st, bool macro) {
+    const bool bgk = c->p.collision == LBM_BGK;
+    if (c->p.storage == LBM_STORE_F32) {
+        if (bgk) particle_launches<StoreF32, COLL_BGK>(c, st, macro);
+        else particle_launches<StoreF32, COLL_MRT>(c, st, macro);
+    } else {
+        if (bgk) particle_launches<StoreF64, COLL_BGK>(c, st, macro);
+        else particle_launches<StoreF64, COLL_MRT>(c, st, macro);
+    }
+}
+
+// GRAPH_STEPS steps without macro output as one graph (captured on first use
+// for the current parities), launched on the context's stream.  The kernels
+// take only pointers fixed for the life of the particle state, so a graph
+// stays valid until the spheres are attached again.
+int replay_graph(lbm_ctx* c) {
+    ParticleState* s = c->part;
+    const int key = 2 * c->cur + s->cur;
+    if (!s->graph[key]) {
+        if (!s->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&s->cap_stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamBeginCapture(s->cap_stream, hipStreamCaptureModeThreadLocal));
+        for (int q = 0; q < GRAPH_STEPS; ++q) launch_step(c, s->cap_stream, false);  // even: parities restored
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamEndCapture(s->cap_stream, &g));
+        hipError_t e = hipGraphInstantiate(&s->graph[key], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            s->graph[key] = nullptr;
+            return lbm_fail(LBM_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        }
+    }
+    HIP_TRY(hipGraphLaunch(s->graph[key], c->stream));
+    return LBM_OK;
+}
+
 void free_state(ParticleState* s) {
+    for (hipGraphExec_t& g : s->graph)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (s->cap_stream) (void)hipStreamDestroy(s->cap_stream);
     for (int q = 0; q < 2; ++q) {
         if (s->xc[q]) (void)hipFree(s->xc[q]);
         if (s->U[q]) (void)hipFree(s->U[q]);
         if (s->Om[q]) (void)hipFree(s->Om[q]);
         if (s->owner[q]) (void)hipFree(s->owner[q]);
     }
-    for (void* p : {(void*)s->r, (void*)s->F, (void*)s->T, (void*)s->Ftot, (void*)s->items, (void*)s->slab_off,
-                    (void*)s->slab_cap, (void*)s->link_count, (void*)s->wave_off, (void*)s->wave_sphere, (void*)s->partial, (void*)s->scal,
-                    (void*)s->red, (void*)s->err, (void*)s->refill, (void*)s->refill_count})
+    for (void* p : {(void*)s->r, (void*)s->F, (void*)s->T, (void*)s->Ftot, (void*)s->items, (void*)s->item_fluid1, (void*)s->slab_off,
+                    (void*)s->slab_cap, (void*)s->scal,
+                    (void*)s->red, (void*)s->err, (void*)s->alone, (void*)s->refill, (void*)s->refill_count})
         if (p) (void)hipFree(p);
 }
 
@@ -978,18 +1072,16 @@ void particles_release(lbm_ctx* c) {
 int particles_step(lbm_ctx* c, int nsteps) {
     ParticleState* s = c->part;
     if (s->stopped) return lbm_fail(LBM_ERR_STATE, "lbm_step: the particle run has stopped; attach the spheres again");
-    const bool bgk = c->p.collision == LBM_BGK;
-    for (int q = 0; q < nsteps; ++q) {
-        const int step = (int)(c->itc + q + 1);
-        const bool macro = q == nsteps - 1;
-        if (c->p.storage == LBM_STORE_F32) {
-            if (bgk) particle_launches<StoreF32, COLL_BGK>(c, step, macro);
-            else particle_launches<StoreF32, COLL_MRT>(c, step, macro);
-        } else {
-            if (bgk) particle_launches<StoreF64, COLL_BGK>(c, step, macro);
-            else particle_launches<StoreF64, COLL_MRT>(c, step, macro);
-        }
+    // the step counter (stop records) restarts from the context's own
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, s->err + 2, (int)c->itc);
+    // graph replays for whole blocks of the steps before the last, direct
+    // launches for the rest; the last step writes the macro fields
+    int q = 0;
+    for (; nsteps - 1 - q >= GRAPH_STEPS; q += GRAPH_STEPS) {
+        int rc = replay_graph(c);
+        if (rc) return rc;
     }
+    for (; q < nsteps; ++q) launch_step(c, c->stream, q == nsteps - 1);
     int rc = lbm_check_launch();
     if (rc) return rc;
     int err[2] = {0, 0};
@@ -1052,7 +1144,6 @@ int lbm_particles_init(lbm_ctx* c, const lbm_particle_params* pp, const double* 
     std::vector<long long> off(pp->np);
     std::vector<int> cap(pp->np);
     long long total_cap = 0;
-    s->max_waves = 0;
     for (int q = 0; q < pp->np; ++q) {
         const double ro = radii[q] + 2.4, ri = radii[q] > 0.9 ? radii[q] - 0.9 : 0.0;
         const double nodes = 4.0 / 3.0 * 3.14159265358979323846 * (ro * ro * ro - ri * ri * ri);
@@ -1060,21 +1151,18 @@ int lbm_particles_init(lbm_ctx* c, const lbm_particle_params* pp, const double* 
         cap[q] = (int)((c18 + 63) / 64 * 64);
         off[q] = total_cap;
         total_cap += cap[q];
-        s->max_waves += cap[q] / 64;
     }
     if (e == hipSuccess) e = hipMalloc(&s->items, sizeof(unsigned) * (size_t)total_cap);
+    if (e == hipSuccess) e = hipMalloc(&s->item_fluid1, (size_t)total_cap);
     if (e == hipSuccess) e = hipMalloc(&s->slab_off, sizeof(long long) * pp->np);
     if (e == hipSuccess) e = hipMalloc(&s->slab_cap, sizeof(int) * pp->np);
-    if (e == hipSuccess) e = hipMalloc(&s->link_count, sizeof(int) * pp->np);
-    if (e == hipSuccess) e = hipMalloc(&s->wave_off, sizeof(int) * (pp->np + 1));
-    if (e == hipSuccess) e = hipMalloc(&s->partial, sizeof(double) * 6 * (size_t)s->max_waves);
-    if (e == hipSuccess) e = hipMalloc(&s->wave_sphere, sizeof(int) * (size_t)s->max_waves);
     if (e == hipSuccess) e = hipMemcpy(s->slab_off, off.data(), sizeof(long long) * pp->np, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->slab_cap, cap.data(), sizeof(int) * pp->np, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(s->link_count, 0, sizeof(int) * pp->np);
     if (e == hipSuccess) e = hipMalloc(&s->scal, sizeof(double) * 2);
     if (e == hipSuccess) e = hipMalloc(&s->red, sizeof(double) * 2 * FLUID_BLOCKS);
-    if (e == hipSuccess) e = hipMalloc(&s->err, 2 * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&s->err, 4 * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&s->alone, sizeof(int) * pp->np);
+
     // uncovered nodes lie inside the old spheres: at most the nodes of all balls
     {
         double cap = 64.0;
@@ -1094,7 +1182,10 @@ int lbm_particles_init(lbm_ctx* c, const lbm_particle_params* pp, const double* 
     if (e == hipSuccess) e = hipMemset(s->F, 0, v3);
     if (e == hipSuccess) e = hipMemset(s->T, 0, v3);
     if (e == hipSuccess) e = hipMemset(s->Ftot, 0, v3);
-    if (e == hipSuccess) e = hipMemset(s->err, 0, 2 * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(s->err, 0, 4 * sizeof(int));
+    // the copies and fills above complete before any launch on the context's
+    // stream (which need not synchronise with the null stream)
+    if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         free_state(s);
         delete s;

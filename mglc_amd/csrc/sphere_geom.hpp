@@ -34,9 +34,17 @@ __host__ __device__ inline double sphere_dist2(double dx, double dy, double dz) 
 // than 2e-9 away from the surface (|d - rad| = |g| / (d + rad), d <= rad + 3
 // on a boundary link): the reference's stop test cannot fire and its branch
 // d > rad is g > 0.  T also covers the rounding of g (below 1e-14 (rad + 2)^2,
-// allowed 1e-12 (rad + 2)^2).  Only inside that band is the point formed and
-// d computed exactly as the reference does, from the point accumulated step
-// by step as the reference accumulates it.
+// allowed 1e-12 (rad + 2)^2).  So the halvings run in two loops: while
+// |g| > T, on g alone (no point, no square root); from the first halving
+// inside the band on, the reference's own test on the point, accumulated
+// step by step as the reference accumulates it.  The point at the switch is
+// X + q e exactly, as the reference's sum is: q then has at most
+// CALQ_FAR_MAX + 1 bits after the binary point, so both are exact for node
+// coordinates below 2^(52 - CALQ_FAR_MAX) = 2^20 (a link enters the band
+// after about 30 halvings; one that has not after CALQ_FAR_MAX goes on in
+// the second loop).
+constexpr int CALQ_FAR_MAX = 32;
+
 __host__ __device__ inline bool calq(double cx, double cy, double cz, double rad, double X, double Y, double Z, int ex,
                                      int ey, int ez, double& x0, double& y0, double& z0, double& qout) {
 #pragma clang fp contract(off)
@@ -46,8 +54,17 @@ __host__ __device__ inline bool calq(double cx, double cy, double cz, double rad
     const double g0 = sphere_dist2(Dx, Dy, Dz) - rad * rad;
     const double band = (2.0 * rad + 3.0) * 2e-9 + 1e-12 * (rad + 2.0) * (rad + 2.0);
     double q = 0.5, qt = 0.5;
-    double px = X + qt * ex, py = Y + qt * ey, pz = Z + qt * ez;
-    for (int it = 0; it < 200; ++it) {
+    int it = 0;
+#pragma unroll 1  // unrolled, the loop holds many more registers for no gain
+    for (; it < CALQ_FAR_MAX; ++it) {
+        const double g = __builtin_fma(__builtin_fma(ga, q, gb2), q, g0);
+        if (fabs(g) <= band) break;
+        qt = qt / 2.0;
+        q += g > 0.0 ? qt : -qt;
+    }
+    double px = X + q * ex, py = Y + q * ey, pz = Z + q * ez;
+#pragma unroll 1
+    for (; it < 200; ++it) {
         const double g = __builtin_fma(__builtin_fma(ga, q, gb2), q, g0);
         bool outside;
         if (fabs(g) <= band) {

@@ -1,6 +1,7 @@
 // calq_check.hip -- host-only check (built with --cuda-host-only, run on the
-// CPU) that lbm::calq, the band-accelerated bisection the particle kernels
-// use, returns bit for bit what the reference's plain loop
+// CPU) that lbm::calq, the two-loop bisection the particle kernels use
+// (halvings on the quadratic alone far from the surface, the reference's own
+// test near it), returns bit for bit what the reference's plain loop
 // (particle_bounceback.F90:101-141, restated below) returns, on every
 // boundary link of many random spheres.  Test infrastructure.
 #include <cstdio>
@@ -47,8 +48,8 @@ int main(int argc, char** argv) {
     long long links = 0, mismatches = 0, band = 0;
     for (int s = 0; s < spheres; ++s) {
         const double rad = s % 7 == 0 ? (double)(2 + s % 5) : 2.0 + 8.0 * uni();  // some integer radii (ties)
-        // centres near the origin and far from it (node coordinates up to ~500)
-        const double base = s % 2 ? 400.0 : 20.0;
+        // centres near the origin and far from it (node coordinates up to ~30000)
+        const double base = s % 3 == 0 ? 20.0 : (s % 3 == 1 ? 400.0 : 30000.0);
         const double cx = base + (s % 11 == 0 ? 0.0 : uni()), cy = base + uni(), cz = base + (s % 13 == 0 ? 0.5 : uni());
         const int b = (int)base;
         for (int k = b - 12; k <= b + 12; ++k)

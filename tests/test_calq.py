@@ -1,9 +1,9 @@
 """CPU check that the wall-distance bisection of the particle kernels
-(mglc_amd/csrc/sphere_geom.hpp: quadratic halvings, exact evaluation only
-near the surface; shared by host and device builds) equals the reference's
-plain calQ loop (particle_bounceback.F90:101-141) bit for bit on every
-boundary link of 300 random spheres (~8e5 links; integer radii, node-centred
-spheres and node coordinates up to ~400 included)."""
+(mglc_amd/csrc/sphere_geom.hpp: halvings on the quadratic alone far from the
+surface, the reference's own test near it; shared by host and device builds)
+equals the reference's plain calQ loop (particle_bounceback.F90:101-141) bit
+for bit on every boundary link of 300 random spheres (~8e5 links; integer
+radii, node-centred spheres and node coordinates up to ~30000 included)."""
 import os
 import shutil
 import subprocess
