@@ -411,7 +411,7 @@ def bench_batch(args, stream_unused):
                           f"U0 U[0.05,0.1], {args.collision.upper()}, FP32 storage",
               "cells": cells, "instances": len(cfgs), "sizes_present": sizes, "storage": "f32",
               "kernel": "k_batch_step (persistent, 100-step hipGraph replays)", "threads": threads,
-              "blocks": blocks, "graph_steps": info.graph_steps}
+              "blocks": blocks, "cells_per_thread": info.cells_per_thread, "graph_steps": info.graph_steps}
     line = line_head(args, mlups, ms / args.steps, "f32", "synthetic (reference initial state per instance)", config)
     # kernel time per launch: the graph holds one step kernel per step and
     # one 1-thread step-counter kernel per 100 steps
