@@ -2,8 +2,6 @@
 // the reference couples discs to D2Q9 (MPI/Micro_particles/fortran/case4/mpi).
 //
 // Per step, all on the device (no host round trip):
-//   k_fluid_sum/mean  rhoAvg over fluid nodes of the last macro
-//                     (particle_bounceback.F90:15-35)
 //   k_links           one block per sphere: the boundary links (fluid node x,
 //                     direction a with x + e_a inside the sphere) of the
 //                     sphere's shell, compacted in a fixed order into the
@@ -16,20 +14,26 @@
 //                     (particle_force.F90:31-92), summed in FP64 in a fixed
 //                     order: force and torque are deterministic
 //   k_particle_step   every node: plain pull streaming with halfway bounce-back
-//                     at the box walls, macro (fluid nodes), collision
+//                     at the box walls, macro (fluid nodes), collision, and
+//                     per block the sums of rho and count over fluid nodes
 //   k_particle_update one wave per sphere: spring and buoyancy forces and the
 //                     kinematics (particle_force.F90:96-191,
 //                     particle_update.F90:26-42); stop codes for
 //                     interpenetration and wall penetration
 //   k_particle_mark   new solid map, rho_s and u = 0 in solids
 //                     (particle_update.F90:69-91)
-//   k_fluid_sum/mean  rhoAvg over the new fluid nodes
 //   k_refill_scan     one block per sphere: its uncovered nodes (old solid,
 //                     new fluid) appended to a list
+//   k_fluid_chain     rhoAvg over the new fluid nodes (particle_bounceback.F90:
+//                     15-35) from the fluid kernel's block sums, corrected for
+//                     the nodes the map update covered and uncovered
 //   k_refill_proc     one lane per listed node: refilled by 3-point
 //                     extrapolation along the most outward lattice direction
 //                     and moment replacement with the body velocity
-//                     (particle_update.F90:110-186), then collided
+//                     (particle_update.F90:110-186), then collided; the change
+//                     of rho goes into the next step's rhoAvg
+// The first step after the spheres are attached takes rhoAvg from a full pass
+// (k_fluid_sum, k_fluid_first).
 // Solid nodes are streamed and collided like fluid ones (their values are
 // never read), which keeps the fluid kernel's waves uniform.
 // The 3-D choices (volume 4/3 pi r^3, I = 2/5 m r^2, springs on all six
@@ -49,6 +53,20 @@
 
 using namespace lbm;
 
+// rhoAvg carried from step to step (particle_bounceback.F90:15-35 takes it
+// over the fluid nodes of the last macro).  After the map update of a step:
+// the fluid kernel's sum over the old map's fluid nodes, less the newly
+// covered nodes (their rho before rho_s), plus rho_s for each uncovered node
+// -- the sum over the new map; the refill then adds (rho - rho_s) for each
+// refilled node -- the sum the next step's streaming uses.
+struct MeanChain {
+    double sum, count;       // fluid sum and count after the map update
+    unsigned long long cov;  // fixed point: rho of the newly covered nodes
+    unsigned long long ref;  // fixed point: rho - rho_s of the refilled nodes
+    int ncov;                // newly covered nodes
+    int pad;
+};
+
 struct ParticleState {
     int np = 0;
     lbm_particle_params pp{};
@@ -67,11 +85,14 @@ struct ParticleState {
     int* refill = nullptr;        // refill_cap pairs (node, sphere) of uncovered nodes
     int* refill_count = nullptr;  // entries this step
     int refill_cap = 0;
-    double* scal = nullptr;  // [0] rhoAvg of the streaming, [1] rhoAvg of the refill
+    double* scal = nullptr;  // rhoAvg of the refill (k_fluid_chain)
     double* red = nullptr;   // reduction partials (k_fluid_sum)
     int* err = nullptr;      // device {stop code (0 = running), step of the stop, number of the step being made,
                              // a sphere moved 1/4 node this step}
     int* alone = nullptr;    // np flags of k_particle_update for k_particle_mark
+    double* cpart = nullptr;      // k_particle_step's per-block fluid sums
+    MeanChain* chain = nullptr;   // rhoAvg carried from step to step
+    bool chain_started = false;   // the first step starts it with a full pass
     bool stopped = false;    // a stop was reported: no further steps
     // graphs of GRAPH_STEPS steps without macro output, one per pair of
     // (fluid buffer, map) parities at their start, captured on cap_stream
@@ -101,9 +122,12 @@ struct PArgs {
     const double *xc, *U, *Om;           // sphere state of this step
     const double *xc_new, *U_new, *Om_new;  // after the update (refill)
     const double* r;
-    const double* rho_avg;  // device scalars: [0] streaming, [1] refill
+    const double* rho_avg;  // rhoAvg of the refill (k_fluid_chain)
     double *rho, *u, *v, *w;
-    int* err;  // {code, step, step counter, some sphere moved 1/4 node}: first stop wins
+    int* err;  // {code, step, steps completed, some sphere moved 1/4 node}: first stop wins
+    int step_off;  // the step a stop reports: steps completed + step_off
+    double* cpart;     // k_particle_step's per-block fluid sums (rho, count)
+    MeanChain* chain;  // rhoAvg from one step to the next
     int* alone;  // per sphere: no other sphere within one node (k_particle_update)
     int np;
     // boundary links
@@ -122,6 +146,14 @@ struct PArgs {
 };
 
 using lbm::calq;
+
+// Fixed point for the fluid-density corrections: values near 1 summed as
+// integers (2^40 per unit), so that sums taken in any order by atomics are
+// the same on every run; a node's rounding is below 1e-12.
+constexpr double FIX_ONE = 1099511627776.0;  // 2^40
+__device__ __forceinline__ unsigned long long to_fix(double x) {
+    return (unsigned long long)__double2ll_rn(x * FIX_ONE);
+}
 
 __device__ __forceinline__ double dist2(double dx, double dy, double dz) { return sphere_dist2(dx, dy, dz); }
 
@@ -145,10 +177,11 @@ __device__ __forceinline__ void body_velocity(const double* xc, const double* U,
     uz = U[3 * id + 2] + wz;
 }
 
-// Record a stop: the first code raised in a step call wins, with its step (the
-// device step counter, advanced by the step's first kernel).
+// Record a stop: the first code raised in a step call wins, with its step:
+// the device count of completed steps (advanced by k_fluid_chain, between
+// the map update and the refill) plus the launch's step_off.
 __device__ __forceinline__ void stop(const PArgs& a, int code) {
-    if (atomicCAS(a.err, 0, code) == 0) atomicExch(a.err + 1, *(volatile const int*)(a.err + 2));
+    if (atomicCAS(a.err, 0, code) == 0) atomicExch(a.err + 1, *(volatile const int*)(a.err + 2) + a.step_off);
 }
 
 // Every particle kernel returns at once after a stop, so the state stays that
@@ -157,6 +190,12 @@ __device__ __forceinline__ bool stopped(const PArgs& a) { return *(volatile cons
 
 __device__ __forceinline__ bool in_box(const PArgs& a, int i, int j, int k) {
     return (unsigned)i < (unsigned)a.nx && (unsigned)j < (unsigned)a.ny && (unsigned)k < (unsigned)a.nz;
+}
+
+// rhoAvg of a step's streaming: the chain after the last step's refill.
+__device__ __forceinline__ double stream_rho_avg(const PArgs& a) {
+    const MeanChain* ch = a.chain;
+    return (ch->sum + (double)(long long)ch->ref / FIX_ONE) / ch->count;
 }
 
 // Absolute population a of a stored state at storage index c.
@@ -233,6 +272,7 @@ constexpr int SCAN_CHUNK = 2048;  // bounding-box nodes tested per block round
 // A sphere's state of this step, read once per block by the link pass.
 struct SphereNow {
     double c[3], u[3], om[3], r;
+    double rho_avg;  // of the fluid, for the moving-wall term
 };
 
 // One boundary link, entry `it` of the sphere `sp` whose bounding box starts at
@@ -304,7 +344,7 @@ __device__ __forceinline__ void link_end(const PArgs& a, const SphereNow& sp, co
                  wz = sp.om[0] * ry0 - sp.om[1] * rx0;  // body_spin
     const double ux = sp.u[0], uy = sp.u[1], uz = sp.u[2];
     const double eu = EX[bd] * (ux + wx) + EY[bd] * (uy + wy) + EZ[bd] * (uz + wz);
-    const double rho_avg = a.rho_avg[0];
+    const double rho_avg = sp.rho_avg;
     double val;
     if (w.near) {
         const double f1 = fl1 ? s1 : weight(al) * a.rho0;
@@ -482,6 +522,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_links(const PArgs a) {
         sp.om[d] = a.Om[3 * id + d];
     }
     sp.r = rad;
+    sp.rho_avg = stream_rho_avg(a);
     for (int e = threadIdx.x; e < pos; e += 2 * SCAN_THREADS) {
         const int e1 = e + SCAN_THREADS;
         const int ee = min(e1, pos - 1);  // past the end: a copy of the last link, computed and dropped
@@ -520,41 +561,68 @@ __global__ __launch_bounds__(256) void k_particle_step(const PArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
-    if (i >= a.nx || j >= a.ny || stopped(a)) return;
-    const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
-    const int c = i + a.px * (j + a.ny * k);
-    const T* __restrict__ src = static_cast<const T*>(a.src);
-    R f[Q];
-    sfor<Q>([&](auto B) {
-        constexpr int bd = decltype(B)::value;
-        constexpr int ex = EX[bd], ey = EY[bd], ez = EZ[bd];
-        if constexpr (bd == 0) {
-            f[0] = S::ld(src + c, a.inv_scale);
-        } else {
-            bool in = true;
-            if constexpr (ex != 0) in = in && (unsigned)(i - ex) < (unsigned)a.nx;
-            if constexpr (ey != 0) in = in && (unsigned)(j - ey) < (unsigned)a.ny;
-            if constexpr (ez != 0) in = in && (unsigned)(k - ez) < (unsigned)a.nz;
-            const long long off = (long long)ex + (long long)ey * a.px + (long long)ez * a.pxy;
-            f[bd] = in ? S::ld(src + (long long)bd * a.ps + (c - off), a.inv_scale)
-                       : S::ld(src + (long long)OPP[bd] * a.ps + c, a.inv_scale);
+    if (stopped(a)) return;  // uniform: no kernel of this launch raises a stop
+    double rho_fluid = 0.0, n_fluid = 0.0;  // this node's part of the fluid sums
+    if (i < a.nx && j < a.ny) {
+        const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
+        const int c = i + a.px * (j + a.ny * k);
+        const T* __restrict__ src = static_cast<const T*>(a.src);
+        R f[Q];
+        sfor<Q>([&](auto B) {
+            constexpr int bd = decltype(B)::value;
+            constexpr int ex = EX[bd], ey = EY[bd], ez = EZ[bd];
+            if constexpr (bd == 0) {
+                f[0] = S::ld(src + c, a.inv_scale);
+            } else {
+                bool in = true;
+                if constexpr (ex != 0) in = in && (unsigned)(i - ex) < (unsigned)a.nx;
+                if constexpr (ey != 0) in = in && (unsigned)(j - ey) < (unsigned)a.ny;
+                if constexpr (ez != 0) in = in && (unsigned)(k - ez) < (unsigned)a.nz;
+                const long long off = (long long)ex + (long long)ey * a.px + (long long)ez * a.pxy;
+                f[bd] = in ? S::ld(src + (long long)bd * a.ps + (c - off), a.inv_scale)
+                           : S::ld(src + (long long)OPP[bd] * a.ps + c, a.inv_scale);
+            }
+        });
+        const Moments<R, S::SHIFT> mo = moments<R, S::SHIFT>(f);
+        if (a.owner[m] < 0) {
+            const double rho = S::SHIFT ? 1.0 + double(mo.drho) : double(mo.rho);
+            a.rho[m] = rho;
+            rho_fluid = rho;
+            n_fluid = 1.0;
+            if constexpr (MACRO) {
+                a.u[m] = double(mo.u);
+                a.v[m] = double(mo.v);
+                a.w[m] = double(mo.w);
+            }
         }
-    });
-    const Moments<R, S::SHIFT> mo = moments<R, S::SHIFT>(f);
-    if (a.owner[m] < 0) {
-        a.rho[m] = S::SHIFT ? 1.0 + double(mo.drho) : double(mo.rho);
-        if constexpr (MACRO) {
-            a.u[m] = double(mo.u);
-            a.v[m] = double(mo.v);
-            a.w[m] = double(mo.w);
-        }
+        collide<S, COLL>(f, mo, a);
+        T* __restrict__ dst = static_cast<T*>(a.dst);
+        sfor<Q>([&](auto B) {
+            constexpr int bd = decltype(B)::value;
+            S::st(dst + (long long)bd * a.ps + c, f[bd], a.scale, 0u);
+        });
     }
-    collide<S, COLL>(f, mo, a);
-    T* __restrict__ dst = static_cast<T*>(a.dst);
-    sfor<Q>([&](auto B) {
-        constexpr int bd = decltype(B)::value;
-        S::st(dst + (long long)bd * a.ps + c, f[bd], a.scale, 0u);
-    });
+    // the block's sums of rho and count over its fluid nodes (fixed order)
+    __shared__ double sh[2][4];
+    const int t = threadIdx.x + blockDim.x * threadIdx.y, lane = t & 63, wid = t >> 6;
+    rho_fluid = wave_sum(rho_fluid);
+    n_fluid = wave_sum(n_fluid);
+    if (lane == 0) {
+        sh[0][wid] = rho_fluid;
+        sh[1][wid] = n_fluid;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const int nw = (int)((blockDim.x * blockDim.y + 63) / 64);
+        double s0 = 0.0, s1 = 0.0;
+        for (int w = 0; w < nw; ++w) {
+            s0 += sh[0][w];
+            s1 += sh[1][w];
+        }
+        const long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+        a.cpart[2 * b] = s0;
+        a.cpart[2 * b + 1] = s1;
+    }
 }
 
 // Streamed population al of node (i, j, k) (in the box), as k_particle_step
@@ -583,7 +651,7 @@ __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
     // p and the stop flag are uniform over the wave (the flag read by lane 0),
     // so the whole wave takes part in the shuffles below
     if (p >= a.np || __builtin_amdgcn_readfirstlane(stopped(a) ? 1 : 0)) return;
-    const double rho_avg = a.rho_avg[0];
+    const double rho_avg = stream_rho_avg(a);
     const double rp = a.r[p];
     const double V = 4.0 / 3.0 * PI * rp * rp * rp;
     double fs[3] = {0.0, 0.0, 0.0};
@@ -687,25 +755,41 @@ __device__ __forceinline__ void for_sphere_nodes(const PArgs& a, const double* x
     }
 }
 
-// New map from the updated spheres; solids at rest with rho_s.
+// New map from the updated spheres; solids at rest with rho_s.  A newly
+// covered node (fluid in the old map) is handled by its first claimer only:
+// its rho (the fluid kernel's, this step) goes into the chain's correction,
+// then rho_s and u = 0 are written.
 __global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner_out) {
     if (stopped(a)) return;
     const int id = blockIdx.x;
     // a node claimed twice goes to the higher index (atomicMax); a sphere that
     // shares no node with another (k_particle_update's test) stores plainly
     const bool plain = a.alone[id] != 0 && *(volatile const int*)(a.err + 3) == 0;
+    unsigned long long cov = 0ull;  // this lane's newly covered nodes, then the wave's
+    int ncov = 0;
     for_sphere_nodes(
         a, a.xc_new, id, [&](long long m) { return a.owner[m]; },
         [&](int, int, int, long long m, int own) {
+            bool first = true;
             if (plain) owner_out[m] = id;
-            else atomicMax(owner_out + m, id);
-            if (own < 0) {  // newly covered; nodes solid before already hold rho_s, u = 0
+            else first = atomicMax(owner_out + m, id) < 0;
+            if (own < 0 && first) {  // nodes solid before already hold rho_s, u = 0
+                cov += to_fix(a.rho[m]);
+                ++ncov;
                 a.rho[m] = a.rho_solid;
                 a.u[m] = 0.0;
                 a.v[m] = 0.0;
                 a.w[m] = 0.0;
             }
         });
+    for (int off = 32; off > 0; off >>= 1) {
+        cov += __shfl_down(cov, off, 64);
+        ncov += __shfl_down(ncov, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && ncov) {
+        atomicAdd(&a.chain->cov, cov);
+        atomicAdd(&a.chain->ncov, ncov);
+    }
 }
 
 // Initial map (from the current state).
@@ -758,7 +842,7 @@ __device__ __forceinline__ void refill_node(const PArgs& a, int i, int j, int k,
     using T = typename S::T;
     using R = typename S::Real;
     const T* src = static_cast<const T*>(a.src);
-    const double rho_avg = a.rho_avg[1];
+    const double rho_avg = *a.rho_avg;
     // extrapolation direction: most outward lattice direction w.r.t. the new centre
     const double rx = i + 1.0 - a.xc_new[3 * id], ry = j + 1.0 - a.xc_new[3 * id + 1],
                  rz = k + 1.0 - a.xc_new[3 * id + 2];
@@ -822,7 +906,9 @@ __device__ __forceinline__ void refill_node(const PArgs& a, int i, int j, int k,
     R f[Q];
     for (int al = 0; al < Q; ++al) f[al] = S::SHIFT ? R(fr[al] - weight(al)) : R(fr[al]);
     const Moments<R, S::SHIFT> mo = moments<R, S::SHIFT>(f);
-    a.rho[m] = S::SHIFT ? 1.0 + double(mo.drho) : double(mo.rho);
+    const double rho_new = S::SHIFT ? 1.0 + double(mo.drho) : double(mo.rho);
+    atomicAdd(&a.chain->ref, to_fix(rho_new - a.rho_solid));
+    a.rho[m] = rho_new;
     a.u[m] = double(mo.u);
     a.v[m] = double(mo.v);
     a.w[m] = double(mo.w);
@@ -895,11 +981,10 @@ __global__ __launch_bounds__(RED_THREADS) void k_fluid_sum(const double* __restr
     }
 }
 
-// rhoAvg = the block partials summed in block order.  Given step_ctr (the
-// first launch of a step), the device step counter moves on and the step's
-// "a sphere moved 1/4 node" flag is cleared.  One block.
-__global__ __launch_bounds__(RED_THREADS) void k_fluid_mean(const double* __restrict__ partial, int nblocks,
-                                                            double* __restrict__ out, int* step_ctr, int* moved) {
+// The first step's rhoAvg from a full pass (k_fluid_sum's partials, summed
+// in block order), which starts the chain.  One block.
+__global__ __launch_bounds__(RED_THREADS) void k_fluid_first(const double* __restrict__ partial, int nblocks,
+                                                             MeanChain* chain) {
     double s = 0.0, cnt = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += RED_THREADS) {
         s += partial[2 * b];
@@ -907,13 +992,54 @@ __global__ __launch_bounds__(RED_THREADS) void k_fluid_mean(const double* __rest
     }
     block_pair_sum(s, cnt);
     if (threadIdx.x == 0) {
-        *out = s / cnt;
-        if (step_ctr) {
-            ++*step_ctr;
-            *moved = 0;
-        }
+        chain->sum = s;
+        chain->count = cnt;
+        chain->cov = chain->ref = 0ull;
+        chain->ncov = 0;
     }
 }
+
+// After the map update and the uncovered-node list: the chain's sum and
+// count over the new map's fluid nodes from the fluid kernel's per-block sums
+// (in block order) and the corrections; rhoAvg of the refill into *out; the
+// corrections cleared for the next step; the count of completed steps moves
+// on and the step's "a sphere moved 1/4 node" flag is cleared.  One block.
+__global__ __launch_bounds__(RED_THREADS) void k_fluid_chain(const PArgs a, long long nblocks, double* out) {
+    if (stopped(a)) return;
+    // eight 16-byte partials in flight per thread and round (the sum is one
+    // block's: its loads must not wait on each other)
+    const double2* cp = reinterpret_cast<const double2*>(a.cpart);
+    double s = 0.0, cnt = 0.0;
+    constexpr int U = 8;
+    for (long long b0 = threadIdx.x; b0 < nblocks; b0 += (long long)U * RED_THREADS) {
+        double2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long b = b0 + (long long)u * RED_THREADS;
+            v[u] = b < nblocks ? cp[b] : double2{0.0, 0.0};
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            s += v[u].x;
+            cnt += v[u].y;
+        }
+    }
+    block_pair_sum(s, cnt);
+    if (threadIdx.x == 0) {
+        MeanChain* ch = a.chain;
+        const int unc = min(*a.refill_count, a.refill_cap);
+        const double sum = s - (double)(long long)ch->cov / FIX_ONE + unc * a.rho_solid;
+        const double count = cnt - ch->ncov + unc;
+        ch->sum = sum;
+        ch->count = count;
+        ch->cov = ch->ref = 0ull;
+        ch->ncov = 0;
+        *out = sum / count;
+        ++a.err[2];
+        a.err[3] = 0;
+    }
+}
+
 
 __global__ void k_fill_int(int* __restrict__ p, long long n, int v) {
     for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x)
@@ -952,6 +1078,8 @@ PArgs particle_args(const lbm_ctx* c) {
     a.w = c->macro + 3 * c->n;
     a.err = s->err;
     a.alone = s->alone;
+    a.cpart = s->cpart;
+    a.chain = s->chain;
     a.np = s->np;
     a.gravity = s->pp.gravity;
     a.threshold_wall = s->pp.threshold_wall;
@@ -975,25 +1103,31 @@ dim3 cell_grid(const lbm_ctx* c) {
     return dim3((c->p.nx + c->block.x - 1) / c->block.x, (c->p.ny + c->block.y - 1) / c->block.y, c->p.nz);
 }
 
-void launch_fluid_mean(lbm_ctx* c, hipStream_t st, const int* owner, double* out, int* step_ctr) {
-    ParticleState* s = c->part;
-    hipLaunchKernelGGL(k_fluid_sum, dim3(FLUID_BLOCKS), dim3(RED_THREADS), 0, st, c->macro, owner, c->n, s->red);
-    hipLaunchKernelGGL(k_fluid_mean, dim3(1), dim3(RED_THREADS), 0, st, s->red, FLUID_BLOCKS, out, step_ctr,
-                       s->err + 3);
-}
-
 __global__ void k_set_int(int* p, int v) { *p = v; }
 
-// The launches of one step on stream st; both parities flip.
+long long cell_blocks(const lbm_ctx* c) {
+    const dim3 g = cell_grid(c);
+    return (long long)g.x * g.y * g.z;
+}
+
+// The launches of one step on stream st; both parities flip.  The first step
+// after the spheres are attached starts the rhoAvg chain with a full pass
+// over the map and the fluid density.
 template <class S, int COLL>
 void particle_launches(lbm_ctx* c, hipStream_t st, bool macro) {
     ParticleState* s = c->part;
     const int np = s->np;
-    launch_fluid_mean(c, st, s->owner[s->cur], s->scal, s->err + 2);
+    if (!s->chain_started) {
+        hipLaunchKernelGGL(k_fluid_sum, dim3(FLUID_BLOCKS), dim3(RED_THREADS), 0, st, c->macro, s->owner[s->cur], c->n,
+                           s->red);
+        hipLaunchKernelGGL(k_fluid_first, dim3(1), dim3(RED_THREADS), 0, st, s->red, FLUID_BLOCKS, s->chain);
+        s->chain_started = true;
+    }
     PArgs a = particle_args(c);
     a.src = c->buf[c->cur];
     a.src_w = c->buf[c->cur];
     a.dst = c->buf[1 - c->cur];
+    a.step_off = 1;  // before k_fluid_chain: the step under way is completed + 1
     hipLaunchKernelGGL(k_links<S>, dim3(np), dim3(SCAN_THREADS), 0, st, a);
     if (macro)
         hipLaunchKernelGGL((k_particle_step<S, COLL, true>), cell_grid(c), c->block, 0, st, a);
@@ -1002,8 +1136,9 @@ void particle_launches(lbm_ctx* c, hipStream_t st, bool macro) {
     hipLaunchKernelGGL(k_particle_update, dim3((np + 3) / 4), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_fill_int, dim3(2048), dim3(256), 0, st, s->owner[1 - s->cur], c->n, -1);
     hipLaunchKernelGGL(k_particle_mark, dim3(np), dim3(256), 0, st, a, s->owner[1 - s->cur]);
-    launch_fluid_mean(c, st, s->owner[1 - s->cur], s->scal + 1, nullptr);
     hipLaunchKernelGGL(k_refill_scan, dim3(np), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_fluid_chain, dim3(1), dim3(RED_THREADS), 0, st, a, cell_blocks(c), s->scal);
+    a.step_off = 0;
     hipLaunchKernelGGL((k_refill_proc<S, COLL>), dim3(REFILL_BLOCKS), dim3(256), 0, st, a);
     s->cur = 1 - s->cur;
     c->cur = 1 - c->cur;
@@ -1056,7 +1191,7 @@ void free_state(ParticleState* s) {
     }
     for (void* p : {(void*)s->r, (void*)s->F, (void*)s->T, (void*)s->Ftot, (void*)s->items, (void*)s->item_fluid1, (void*)s->slab_off,
                     (void*)s->slab_cap, (void*)s->scal,
-                    (void*)s->red, (void*)s->err, (void*)s->alone, (void*)s->refill, (void*)s->refill_count})
+                    (void*)s->red, (void*)s->err, (void*)s->alone, (void*)s->cpart, (void*)s->chain, (void*)s->refill, (void*)s->refill_count})
         if (p) (void)hipFree(p);
 }
 
@@ -1072,11 +1207,15 @@ void particles_release(lbm_ctx* c) {
 int particles_step(lbm_ctx* c, int nsteps) {
     ParticleState* s = c->part;
     if (s->stopped) return lbm_fail(LBM_ERR_STATE, "lbm_step: the particle run has stopped; attach the spheres again");
-    // the step counter (stop records) restarts from the context's own
+    // the count of completed steps (stop records) restarts from the context's own
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, s->err + 2, (int)c->itc);
     // graph replays for whole blocks of the steps before the last, direct
     // launches for the rest; the last step writes the macro fields
     int q = 0;
+    if (!s->chain_started && nsteps > 0) {  // the full pass that starts the rhoAvg chain stays out of the graphs
+        launch_step(c, c->stream, nsteps == 1);
+        q = 1;
+    }
     for (; nsteps - 1 - q >= GRAPH_STEPS; q += GRAPH_STEPS) {
         int rc = replay_graph(c);
         if (rc) return rc;
@@ -1158,10 +1297,13 @@ int lbm_particles_init(lbm_ctx* c, const lbm_particle_params* pp, const double* 
     if (e == hipSuccess) e = hipMalloc(&s->slab_cap, sizeof(int) * pp->np);
     if (e == hipSuccess) e = hipMemcpy(s->slab_off, off.data(), sizeof(long long) * pp->np, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->slab_cap, cap.data(), sizeof(int) * pp->np, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&s->scal, sizeof(double) * 2);
+    if (e == hipSuccess) e = hipMalloc(&s->scal, sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&s->red, sizeof(double) * 2 * FLUID_BLOCKS);
     if (e == hipSuccess) e = hipMalloc(&s->err, 4 * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&s->alone, sizeof(int) * pp->np);
+    if (e == hipSuccess) e = hipMalloc(&s->cpart, sizeof(double) * 2 * (size_t)cell_blocks(c));
+    if (e == hipSuccess) e = hipMalloc(&s->chain, sizeof(MeanChain));
+    if (e == hipSuccess) e = hipMemset(s->chain, 0, sizeof(MeanChain));
 
     // uncovered nodes lie inside the old spheres: at most the nodes of all balls
     {
