@@ -91,6 +91,7 @@ struct ParticleState {
                              // a sphere moved 1/4 node this step}
     int* alone = nullptr;    // np flags of k_particle_update for k_particle_mark
     double* cpart = nullptr;      // k_particle_step's per-block fluid sums
+    double* cpart2 = nullptr;     // their slice sums (k_refill_scan)
     MeanChain* chain = nullptr;   // rhoAvg carried from step to step
     bool chain_started = false;   // the first step starts it with a full pass
     bool stopped = false;    // a stop was reported: no further steps
@@ -104,8 +105,12 @@ namespace {
 
 constexpr double PI = 3.14159265358979323846;
 constexpr int REFILL_BLOCKS = 128;  // k_refill_proc: lanes walk the list with a grid stride
-constexpr int GRAPH_STEPS = 10;    // steps per graph replay; even, so the buffers end where they started
+constexpr int GRAPH_STEPS = 10;
+constexpr int SUM_BLOCKS = 256;      // k_refill_scan: at least this many blocks (slices of the fluid sums)
+constexpr int UPDATE_CHUNK = 1024;  // k_particle_update: spheres staged in LDS at a time (32 KiB)    // steps per graph replay; even, so the buffers end where they started
 constexpr int FLUID_BLOCKS = 4096;  // k_fluid_sum: enough waves in flight to stream the map and rho
+
+inline int scan_blocks(int np) { return np > SUM_BLOCKS ? np : SUM_BLOCKS; }
 
 struct PArgs {
     const void* src;
@@ -127,6 +132,8 @@ struct PArgs {
     int* err;  // {code, step, steps completed, some sphere moved 1/4 node}: first stop wins
     int step_off;  // the step a stop reports: steps completed + step_off
     double* cpart;     // k_particle_step's per-block fluid sums (rho, count)
+    double* cpart2;    // their slice sums, one pair per k_refill_scan block
+    long long ncb;     // k_particle_step's blocks
     MeanChain* chain;  // rhoAvg from one step to the next
     int* alone;  // per sphere: no other sphere within one node (k_particle_update)
     int np;
@@ -645,39 +652,59 @@ __device__ __forceinline__ double streamed_pop_at(const PArgs& a, const typename
 // the wave sums their parts (fixed shuffle tree), lane 0 adds the wall
 // springs and buoyancy and advances the sphere.
 __global__ __launch_bounds__(256) void k_particle_update(const PArgs a) {
+    // the other spheres' centres and radii, UPDATE_CHUNK at a time, staged in
+    // LDS by the whole block (coalesced) and read by each wave's lanes
+    __shared__ double4 others[UPDATE_CHUNK];
     const int lane = threadIdx.x & 63;
     const int p = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.refill_count = 0;  // the refill list of this step starts empty
     // p and the stop flag are uniform over the wave (the flag read by lane 0),
-    // so the whole wave takes part in the shuffles below
-    if (p >= a.np || __builtin_amdgcn_readfirstlane(stopped(a) ? 1 : 0)) return;
+    // so the whole wave takes part in the shuffles below; every wave takes
+    // part in the staging barriers
+    const bool live = p < a.np && !__builtin_amdgcn_readfirstlane(stopped(a) ? 1 : 0);
     const double rho_avg = stream_rho_avg(a);
-    const double rp = a.r[p];
+    const int pp = live ? p : 0;
+    const double rp = a.r[pp];
     const double V = 4.0 / 3.0 * PI * rp * rp * rp;
     double fs[3] = {0.0, 0.0, 0.0};
     const double scale_p = V * (a.rho_solid - rho_avg) * a.gravity / a.stiff_particle;
-    const double px = a.xc[3 * p], py = a.xc[3 * p + 1], pz = a.xc[3 * p + 2];
+    const double px = a.xc[3 * pp], py = a.xc[3 * pp + 1], pz = a.xc[3 * pp + 2];
     bool close = false;  // another sphere within one node of this one
-    for (int q = lane; q < a.np; q += 64) {
-        if (q == p) continue;
-        const double dx = px - a.xc[3 * q], dy = py - a.xc[3 * q + 1], dz = pz - a.xc[3 * q + 2];
-        const double rs = rp + a.r[q];
-        const double r1 = rs + 1.0;
-        if (fabs(dx) < r1 && fabs(dy) < r1 && fabs(dz) < r1 && dist2(dx, dy, dz) < r1 * r1) close = true;
-        const double reach = rs + a.threshold_particle;
-        if (fabs(dx) >= reach || fabs(dy) >= reach || fabs(dz) >= reach) continue;  // dij >= reach too
-        const double dij = sqrt(dist2(dx, dy, dz));
-        if (dij >= reach) continue;
-        if (dij < rs) {
-            stop(a, 1);
-            continue;
+    for (int q0 = 0; q0 < a.np; q0 += UPDATE_CHUNK) {
+        const int len = min(UPDATE_CHUNK, a.np - q0);
+        __syncthreads();  // the previous chunk is read
+        for (int t = threadIdx.x; t < len; t += blockDim.x) {
+            const int q = q0 + t;
+            others[t] = double4{a.xc[3 * q], a.xc[3 * q + 1], a.xc[3 * q + 2], a.r[q]};
         }
-        const double g = (dij - rp - a.r[q] - a.threshold_particle) / a.threshold_particle;
-        const double mag = scale_p * (g * g);
-        fs[0] += mag * dx / dij;
-        fs[1] += mag * dy / dij;
-        fs[2] += mag * dz / dij;
+        __syncthreads();
+        if (!live) continue;
+        // lane by lane over q = lane, lane + 64, ... as one pass over all
+        // spheres would (chunks are whole multiples of 64)
+        for (int t = lane; t < len; t += 64) {
+            const int q = q0 + t;
+            if (q == p) continue;
+            const double4 o = others[t];
+            const double dx = px - o.x, dy = py - o.y, dz = pz - o.z;
+            const double rs = rp + o.w;
+            const double r1 = rs + 1.0;
+            if (fabs(dx) < r1 && fabs(dy) < r1 && fabs(dz) < r1 && dist2(dx, dy, dz) < r1 * r1) close = true;
+            const double reach = rs + a.threshold_particle;
+            if (fabs(dx) >= reach || fabs(dy) >= reach || fabs(dz) >= reach) continue;  // dij >= reach too
+            const double dij = sqrt(dist2(dx, dy, dz));
+            if (dij >= reach) continue;
+            if (dij < rs) {
+                stop(a, 1);
+                continue;
+            }
+            const double g = (dij - rp - o.w - a.threshold_particle) / a.threshold_particle;
+            const double mag = scale_p * (g * g);
+            fs[0] += mag * dx / dij;
+            fs[1] += mag * dy / dij;
+            fs[2] += mag * dz / dij;
+        }
     }
+    if (!live) return;
     for (int d = 0; d < 3; ++d) fs[d] = wave_sum(fs[d]);
     const bool alone = __ballot(close) == 0ull;
     if (lane != 0) return;
@@ -921,21 +948,38 @@ __device__ __forceinline__ void refill_node(const PArgs& a, int i, int j, int k,
 // The uncovered nodes of each sphere, appended to the refill list.  The
 // list's order does not matter: a node's refill reads only the previous step's
 // populations, the maps and the sphere states, and writes only its own cell.
-__global__ __launch_bounds__(256) void k_refill_scan(const PArgs a) {
+// Every block also sums its slice of the fluid kernel's per-block sums
+// (block order within the slice) into cpart2, for k_fluid_chain; the grid has
+// at least SUM_BLOCKS blocks so that the slices stay short.
+__global__ __launch_bounds__(RED_THREADS) void k_refill_scan(const PArgs a) {
     if (stopped(a)) return;
     const int id = blockIdx.x;
-    for_sphere_nodes(
-        a, a.xc, id, [&](long long m) { return int2{a.owner[m], a.owner_new[m]}; },
-        [&](int, int, int, long long m, int2 own) {
-            if (!(own.x == id && own.y < 0)) return;
-            const int slot = atomicAdd(a.refill_count, 1);
-            if (slot < a.refill_cap) {
-                a.refill[2 * slot] = (int)m;
-                a.refill[2 * slot + 1] = id;
-            } else {
-                stop(a, 6);
-            }
-        });
+    if (id < a.np)
+        for_sphere_nodes(
+            a, a.xc, id, [&](long long m) { return int2{a.owner[m], a.owner_new[m]}; },
+            [&](int, int, int, long long m, int2 own) {
+                if (!(own.x == id && own.y < 0)) return;
+                const int slot = atomicAdd(a.refill_count, 1);
+                if (slot < a.refill_cap) {
+                    a.refill[2 * slot] = (int)m;
+                    a.refill[2 * slot + 1] = id;
+                } else {
+                    stop(a, 6);
+                }
+            });
+    const long long lo = a.ncb * blockIdx.x / gridDim.x, hi = a.ncb * (blockIdx.x + 1) / gridDim.x;
+    const double2* cp = reinterpret_cast<const double2*>(a.cpart);
+    double s = 0.0, cnt = 0.0;
+    for (long long b = lo + threadIdx.x; b < hi; b += RED_THREADS) {
+        const double2 v = cp[b];
+        s += v.x;
+        cnt += v.y;
+    }
+    block_pair_sum(s, cnt);
+    if (threadIdx.x == 0) {
+        a.cpart2[2 * blockIdx.x] = s;
+        a.cpart2[2 * blockIdx.x + 1] = cnt;
+    }
 }
 
 template <class S, int COLL>
@@ -1001,28 +1045,15 @@ __global__ __launch_bounds__(RED_THREADS) void k_fluid_first(const double* __res
 
 // After the map update and the uncovered-node list: the chain's sum and
 // count over the new map's fluid nodes from the fluid kernel's per-block sums
-// (in block order) and the corrections; rhoAvg of the refill into *out; the
+// (k_refill_scan's slice sums, in order) and the corrections; rhoAvg of the refill into *out; the
 // corrections cleared for the next step; the count of completed steps moves
 // on and the step's "a sphere moved 1/4 node" flag is cleared.  One block.
-__global__ __launch_bounds__(RED_THREADS) void k_fluid_chain(const PArgs a, long long nblocks, double* out) {
+__global__ __launch_bounds__(RED_THREADS) void k_fluid_chain(const PArgs a, int nslices, double* out) {
     if (stopped(a)) return;
-    // eight 16-byte partials in flight per thread and round (the sum is one
-    // block's: its loads must not wait on each other)
-    const double2* cp = reinterpret_cast<const double2*>(a.cpart);
     double s = 0.0, cnt = 0.0;
-    constexpr int U = 8;
-    for (long long b0 = threadIdx.x; b0 < nblocks; b0 += (long long)U * RED_THREADS) {
-        double2 v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const long long b = b0 + (long long)u * RED_THREADS;
-            v[u] = b < nblocks ? cp[b] : double2{0.0, 0.0};
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            s += v[u].x;
-            cnt += v[u].y;
-        }
+    for (int b = threadIdx.x; b < nslices; b += RED_THREADS) {
+        s += a.cpart2[2 * b];
+        cnt += a.cpart2[2 * b + 1];
     }
     block_pair_sum(s, cnt);
     if (threadIdx.x == 0) {
@@ -1044,6 +1075,15 @@ __global__ __launch_bounds__(RED_THREADS) void k_fluid_chain(const PArgs a, long
 __global__ void k_fill_int(int* __restrict__ p, long long n, int v) {
     for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x)
         p[c] = v;
+}
+
+dim3 cell_grid(const lbm_ctx* c) {
+    return dim3((c->p.nx + c->block.x - 1) / c->block.x, (c->p.ny + c->block.y - 1) / c->block.y, c->p.nz);
+}
+
+long long cell_blocks(const lbm_ctx* c) {
+    const dim3 g = cell_grid(c);
+    return (long long)g.x * g.y * g.z;
 }
 
 PArgs particle_args(const lbm_ctx* c) {
@@ -1079,6 +1119,8 @@ PArgs particle_args(const lbm_ctx* c) {
     a.err = s->err;
     a.alone = s->alone;
     a.cpart = s->cpart;
+    a.cpart2 = s->cpart2;
+    a.ncb = cell_blocks(c);
     a.chain = s->chain;
     a.np = s->np;
     a.gravity = s->pp.gravity;
@@ -1099,16 +1141,9 @@ PArgs particle_args(const lbm_ctx* c) {
     return a;
 }
 
-dim3 cell_grid(const lbm_ctx* c) {
-    return dim3((c->p.nx + c->block.x - 1) / c->block.x, (c->p.ny + c->block.y - 1) / c->block.y, c->p.nz);
-}
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
 
-long long cell_blocks(const lbm_ctx* c) {
-    const dim3 g = cell_grid(c);
-    return (long long)g.x * g.y * g.z;
-}
 
 // The launches of one step on stream st; both parities flip.  The first step
 // after the spheres are attached starts the rhoAvg chain with a full pass
@@ -1136,8 +1171,8 @@ void particle_launches(lbm_ctx* c, hipStream_t st, bool macro) {
     hipLaunchKernelGGL(k_particle_update, dim3((np + 3) / 4), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_fill_int, dim3(2048), dim3(256), 0, st, s->owner[1 - s->cur], c->n, -1);
     hipLaunchKernelGGL(k_particle_mark, dim3(np), dim3(256), 0, st, a, s->owner[1 - s->cur]);
-    hipLaunchKernelGGL(k_refill_scan, dim3(np), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(k_fluid_chain, dim3(1), dim3(RED_THREADS), 0, st, a, cell_blocks(c), s->scal);
+    hipLaunchKernelGGL(k_refill_scan, dim3(scan_blocks(np)), dim3(RED_THREADS), 0, st, a);
+    hipLaunchKernelGGL(k_fluid_chain, dim3(1), dim3(RED_THREADS), 0, st, a, scan_blocks(np), s->scal);
     a.step_off = 0;
     hipLaunchKernelGGL((k_refill_proc<S, COLL>), dim3(REFILL_BLOCKS), dim3(256), 0, st, a);
     s->cur = 1 - s->cur;
@@ -1191,7 +1226,7 @@ void free_state(ParticleState* s) {
     }
     for (void* p : {(void*)s->r, (void*)s->F, (void*)s->T, (void*)s->Ftot, (void*)s->items, (void*)s->item_fluid1, (void*)s->slab_off,
                     (void*)s->slab_cap, (void*)s->scal,
-                    (void*)s->red, (void*)s->err, (void*)s->alone, (void*)s->cpart, (void*)s->chain, (void*)s->refill, (void*)s->refill_count})
+                    (void*)s->red, (void*)s->err, (void*)s->alone, (void*)s->cpart, (void*)s->cpart2, (void*)s->chain, (void*)s->refill, (void*)s->refill_count})
         if (p) (void)hipFree(p);
 }
 
@@ -1302,6 +1337,7 @@ int lbm_particles_init(lbm_ctx* c, const lbm_particle_params* pp, const double* 
     if (e == hipSuccess) e = hipMalloc(&s->err, 4 * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&s->alone, sizeof(int) * pp->np);
     if (e == hipSuccess) e = hipMalloc(&s->cpart, sizeof(double) * 2 * (size_t)cell_blocks(c));
+    if (e == hipSuccess) e = hipMalloc(&s->cpart2, sizeof(double) * 2 * (size_t)scan_blocks(pp->np));
     if (e == hipSuccess) e = hipMalloc(&s->chain, sizeof(MeanChain));
     if (e == hipSuccess) e = hipMemset(s->chain, 0, sizeof(MeanChain));
 
