@@ -433,12 +433,36 @@ struct Table {
         {k_cavity_interior<S, COLL_BGK, false>, k_cavity_interior<S, COLL_BGK, true>}};
 };
 
+// FP16 MRT at the default store scale 2^12, folded into the inverse
+// transform (StoreF16K12): the same bits as StoreF16, 19 multiplies fewer
+// per cell.
+struct TableK12 {
+    using S = StoreF16K12;
+    // step[macro][halo][loop]
+    static constexpr StepFn step[2][2][2] = {
+        {{k_cavity_step<S, COLL_MRT, false, false, false>, k_cavity_step<S, COLL_MRT, false, false, true>},
+         {k_cavity_step<S, COLL_MRT, false, true, false>, k_cavity_step<S, COLL_MRT, false, true, true>}},
+        {{k_cavity_step<S, COLL_MRT, true, false, false>, k_cavity_step<S, COLL_MRT, true, false, true>},
+         {k_cavity_step<S, COLL_MRT, true, true, false>, k_cavity_step<S, COLL_MRT, true, true, true>}}};
+    static constexpr ShellFn shell[2][2] = {
+        {k_cavity_shell<S, COLL_MRT, false, false>, k_cavity_shell<S, COLL_MRT, false, true>},
+        {k_cavity_shell<S, COLL_MRT, true, false>, k_cavity_shell<S, COLL_MRT, true, true>}};
+    static constexpr InteriorFn interior[2] = {k_cavity_interior<S, COLL_MRT, false>,
+                                               k_cavity_interior<S, COLL_MRT, true>};
+};
+
 }  // namespace
 
 
 namespace {
 
 void loopback_masks(const lbm_ctx* c, unsigned* pack, unsigned* unpack);
+
+// FP16-stored MRT on the VALU at scale 2^12: the folded-scale kernels.
+bool folds_scale(const lbm_ctx* c) {
+    return c->p.storage == LBM_STORE_F16 && c->p.collision != LBM_BGK && c->p.transform != 2 &&
+           c->scale == 4096.0f;
+}
 
 StepArgs make_args(const lbm_ctx* c) {
     StepArgs a{};
@@ -482,6 +506,7 @@ StepFn step_fn(const lbm_ctx* c, bool macro, bool loop) {
     case LBM_STORE_F32: return Table<StoreF32>::step[coll][macro][halo][loop];
     case LBM_STORE_F16:
         if (coll == COLL_MRT && c->p.transform == 2) return Table<StoreF16>::stepx[macro][halo][loop];
+        if (folds_scale(c)) return TableK12::step[macro][halo][loop];
         return Table<StoreF16>::step[coll][macro][halo][loop];
     default: return Table<StoreF64>::step[coll][macro][halo][loop];
     }
@@ -601,7 +626,9 @@ int launch_shell(lbm_ctx* c, const StepArgs& a, bool macro, bool loop) {
     ShellFn fn;
     switch (c->p.storage) {
     case LBM_STORE_F32: fn = Table<StoreF32>::shell[coll][macro][loop]; break;
-    case LBM_STORE_F16: fn = Table<StoreF16>::shell[coll][macro][loop]; break;
+    case LBM_STORE_F16:
+        fn = folds_scale(c) ? TableK12::shell[macro][loop] : Table<StoreF16>::shell[coll][macro][loop];
+        break;
     default: fn = Table<StoreF64>::shell[coll][macro][loop];
     }
     const int bx = c->block.x;
@@ -618,7 +645,7 @@ int launch_interior(lbm_ctx* c, const StepArgs& a, bool macro) {
     InteriorFn fn;
     switch (c->p.storage) {
     case LBM_STORE_F32: fn = Table<StoreF32>::interior[coll][macro]; break;
-    case LBM_STORE_F16: fn = Table<StoreF16>::interior[coll][macro]; break;
+    case LBM_STORE_F16: fn = folds_scale(c) ? TableK12::interior[macro] : Table<StoreF16>::interior[coll][macro]; break;
     default: fn = Table<StoreF64>::interior[coll][macro];
     }
     const dim3 grid((c->p.nx + c->block.x - 1) / c->block.x, (j1 - j0 + c->block.y - 1) / c->block.y, k1 - k0);

@@ -45,6 +45,7 @@ __device__ __forceinline__ R mac(R acc, R x) {
 // 2-4% faster at 256^3).
 // Every st() takes a 32-bit random word; only the FP16 format uses it.
 struct StoreF64 {
+    static constexpr int FOLD_LOG2 = 0;  // the store scale the MRT inverse folds in (0: none)
     using T = double;
     using Real = double;
     static constexpr bool SHIFT = false;
@@ -57,6 +58,7 @@ struct StoreF64 {
 };
 
 struct StoreF32 {
+    static constexpr int FOLD_LOG2 = 0;  // the store scale the MRT inverse folds in (0: none)
     using T = float;
     using Real = float;
     static constexpr bool SHIFT = true;
@@ -95,6 +97,7 @@ __device__ __forceinline__ unsigned mix32(unsigned x) {
 // IEEE binary16 holding (f - w) * 2^k.  The scale is an exact power of two,
 // so scaling adds no rounding; the store rounds stochastically.
 struct StoreF16 {
+    static constexpr int FOLD_LOG2 = 0;  // the store scale the MRT inverse folds in (0: none)
     using T = _Float16;
     using Real = float;
     static constexpr bool SHIFT = true;
@@ -111,6 +114,17 @@ struct StoreF16 {
     __device__ __forceinline__ static T cvt(Real v, float scale, unsigned rnd) {
         return to_half_stochastic(v * scale, rnd);
     }
+    __device__ __forceinline__ static void st(T* p, Real v, float scale, unsigned rnd) {
+        __builtin_nontemporal_store(cvt(v, scale, rnd), p);
+    }
+};
+
+// FP16 storage at the default scale 2^12 with the scale folded into the MRT
+// inverse transform's coefficients (exact: a power of two), so the store
+// converts without a multiply; the bits equal StoreF16's at scale 2^12.
+struct StoreF16K12 : StoreF16 {
+    static constexpr int FOLD_LOG2 = 12;
+    __device__ __forceinline__ static T cvt(Real v, float, unsigned rnd) { return to_half_stochastic(v, rnd); }
     __device__ __forceinline__ static void st(T* p, Real v, float scale, unsigned rnd) {
         __builtin_nontemporal_store(cvt(v, scale, rnd), p);
     }
@@ -218,14 +232,17 @@ __device__ __forceinline__ void mrt_forward(const R f[Q], R m[Q]) {
     });
 }
 
-template <class R>
+// FOLD: the result times 2^FOLD, the scale folded into the coefficients
+// (exact for a power of two: the same bits as scaling the result).
+template <class R, int FOLD = 0>
 __device__ __forceinline__ void mrt_inverse(const R m[Q], R f[Q]) {
+    constexpr double SC = double(1LL << FOLD);
     {
         R acc = R(-0.0);
         sfor<Q>([&](auto K) {
             constexpr int k = decltype(K)::value;
             constexpr int c = basis(k, 0);
-            if constexpr (c != 0) acc = acc + R(double(c) / double(row_norm2(k))) * m[k];
+            if constexpr (c != 0) acc = acc + R(double(c) / double(row_norm2(k)) * SC) * m[k];
         });
         f[0] = acc;
     }
@@ -236,7 +253,7 @@ __device__ __forceinline__ void mrt_inverse(const R m[Q], R f[Q]) {
             constexpr int k = decltype(K)::value;
             constexpr int c = basis(k, a);
             if constexpr (c != 0) {
-                constexpr R coef = R(double(c) / double(row_norm2(k)));
+                constexpr R coef = R(double(c) / double(row_norm2(k)) * SC);
                 if constexpr (row_is_odd(k)) o = o + coef * m[k];
                 else e = e + coef * m[k];
             }
@@ -288,7 +305,7 @@ constexpr int rate_kind() {
     return (k == 0 || k == 3 || k == 5 || k == 7) ? 0 : ((k == 4 || k == 6 || k == 8 || k >= 16) ? 2 : 1);
 }
 
-template <class R, bool SHIFT>
+template <class R, bool SHIFT, int FOLD = 0>
 __device__ __forceinline__ void collide_mrt(R f[Q], const Moments<R, SHIFT>& q, R snu, R sq,
                                             bool meq12_rho) {
     R m[Q], meq[Q];
@@ -300,7 +317,7 @@ __device__ __forceinline__ void collide_mrt(R f[Q], const Moments<R, SHIFT>& q, 
         if constexpr (kind == 1) m[k] = m[k] - snu * (m[k] - meq[k]);
         else if constexpr (kind == 2) m[k] = m[k] - sq * (m[k] - meq[k]);
     });
-    mrt_inverse(m, f);
+    mrt_inverse<R, FOLD>(m, f);
 }
 
 }  // namespace lbm

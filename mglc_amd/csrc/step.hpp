@@ -56,10 +56,13 @@ __device__ __forceinline__ void collide(typename S::Real f[Q],
                                         const Moments<typename S::Real, S::SHIFT>& q,
                                         const Args& a) {
     using R = typename S::Real;
-    if constexpr (COLL == COLL_BGK)
+    if constexpr (COLL == COLL_BGK) {
+        static_assert(S::FOLD_LOG2 == 0, "a folded store scale is an MRT form");
         collide_bgk<R, S::SHIFT>(f, q, R(a.snu));
-    else
-        collide_mrt<R, S::SHIFT>(f, q, R(a.snu), R(a.sq), a.meq12_rho != 0);
+    } else {
+        // f returned times 2^FOLD_LOG2 when the store folds its scale in
+        collide_mrt<R, S::SHIFT, S::FOLD_LOG2>(f, q, R(a.snu), R(a.sq), a.meq12_rho != 0);
+    }
 }
 
 // Value pulled from outside the domain for population al at cell (i, j, k):
