@@ -24,6 +24,11 @@ Other workloads (--workload), one JSON line each in the same format:
              MRT temperature, Ra = 1e5, Pr = 0.71, walls at +-0.5, seeded
              noise; Nusselt numbers after --accuracy-steps
 
+With the default workload, `other_workloads` in the same line holds short
+runs of the other three (1000 timed steps each, no CPU legs, no probes):
+their MLUPS, step time and roofline as measured on the same GPU in the same
+process (--other-workloads 0 leaves them out).
+
 Prints ONE JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -74,6 +79,9 @@ def parse():
     ap.add_argument("--device-warmup", type=int, default=1,
                     help="cavity / thermal: before the W warmup steps, step a throwaway context of the same configuration "
                          "for ~150 ms of device time (steady clocks; the measured state is untouched)")
+    ap.add_argument("--other-workloads", type=int, default=1,
+                    help="with the cavity workload: also time the batch, particle and buoyancy workloads briefly "
+                         "and report them under other_workloads")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -623,6 +631,32 @@ def bench_thermal(args, stream):
     return line
 
 
+# the short runs of the other workloads beside the headline line: timed
+# steps, warm-up steps, cells per edge, accuracy steps (0: compared at the
+# end of the timed run)
+OTHER_WORKLOADS = {
+    "particles": (1000, 20, 192, 0),
+    "thermal": (1000, 50, 128, 0),
+    "batch": (1000, 200, 0, 0),
+}
+
+
+def other_workloads(args, stream):
+    out = {}
+    for name, (steps, warmup, n, acc) in OTHER_WORKLOADS.items():
+        a = argparse.Namespace(**vars(args))
+        a.workload, a.steps, a.warmup, a.n, a.accuracy_steps = name, steps, warmup, n, acc
+        a.cpu_baseline, a.attainable = 0, 0
+        t0 = time.perf_counter()
+        line = {"particles": bench_particles, "thermal": bench_thermal, "batch": bench_batch}[name](a, stream)
+        keep = {k: line[k] for k in ("value", "unit", "ms_per_step", "dtype", "steps", "warmup", "config",
+                                     "roofline", "fp64", "accuracy", "nusselt", "variants", "one_at_a_time")
+                if k in line}
+        keep["wall_s"] = round(time.perf_counter() - t0, 1)
+        out[name] = keep
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -640,6 +674,8 @@ def main():
     fn = {"cavity": bench_cavity, "batch": bench_batch, "particles": bench_particles,
           "thermal": bench_thermal}[args.workload]
     line = fn(args, stream)
+    if args.workload == "cavity" and args.other_workloads:
+        line["other_workloads"] = other_workloads(args, stream)
     stream.close()
     print(json.dumps(line), flush=True)
 
