@@ -213,7 +213,6 @@ class CavityConfig:
         p.block_x, p.block_y = self.block_x, self.block_y
         p.halo_faces, p.halo_loopback, p.halo_overlap = self.halo_faces, self.halo_loopback, self.halo_overlap
         p.waves_per_cu = self.waves_per_cu
-        p.halo_faces, p.halo_loopback, p.halo_overlap = self.halo_faces, self.halo_loopback, self.halo_overlap
         p.transform = TRANSFORM_NAMES[self.transform]
         p.tau, p.u_lid, p.rho0 = self.tau(), self.U0, self.rho0
         return p
