@@ -59,13 +59,17 @@ struct ThermalArgs {
 template <class S>
 using ForceT = std::conditional_t<std::is_same_v<S, StoreF64>, double, float>;
 
-// A stored population read by the buoyancy step: non-temporal.  Measured on
-// MI355X at 128^3: FP16 62.9 -> 58.9 us per step, FP64 and FP32 unchanged
-// (171.8 and 82.5 us); the cavity, batch and particle kernels ran 2-17 %
-// slower with such loads, so they keep ordinary ones.
+// A stored population read by the buoyancy step: non-temporal in FP16
+// storage.  Measured on MI355X at 128^3: FP16 62.9 -> 58.9 us per step;
+// FP64 and FP32 ran no faster (171.8 and 82.5 us) and fetched 4 % more
+// (lines shared by neighbouring blocks evicted sooner), and the cavity, batch
+// and particle kernels ran 2-17 % slower, so those keep ordinary loads.
 template <class S>
 __device__ __forceinline__ typename S::Real ld_stream(const typename S::T* p, float inv_scale) {
-    return S::val(__builtin_nontemporal_load(p), inv_scale);
+    if constexpr (S::BYTES == 2)
+        return S::val(__builtin_nontemporal_load(p), inv_scale);
+    else
+        return S::ld(p, inv_scale);
 }
 
 // One step of the buoyancy-driven cavity.  S = StoreF64: FP64 arithmetic in
