@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_batch_step(const BatchArgs b) {
         a.v = b.v + in.macro_off;
         a.w = b.w + in.macro_off;
         a.seed = seed;
-        cavity_cell<S, COLL, MACRO, false>(a, (int)i, (int)j, (int)k);
+        cavity_cell<S, COLL, MACRO, false, false, true>(a, (int)i, (int)j, (int)k);
     }
 }
 

@@ -196,7 +196,11 @@ __device__ __forceinline__ void mfma_cell(const Args& a, const _Float16* src, _F
 // way and every kernel built on this function produces the same bits.
 // MX: the forward MRT moment transform of whole FP16 wavefronts away from the
 // y and z faces on the matrix cores (mfma.hpp); other waves use the VALU path.
-template <class S, int COLL, bool MACRO, bool HALO, bool MX = false, class Args, class Post = NoPost>
+// XSEL: a lane on an x face points its unconditional x-shifted pulls at its
+// own column of the pulled row (the value is replaced by the wall's), so the
+// wave reads no line of a neighbouring row; for the short rows of the batched
+// instances (32-96 cells) that saves a line per row and plane.
+template <class S, int COLL, bool MACRO, bool HALO, bool MX = false, bool XSEL = false, class Args, class Post = NoPost>
 __device__ __forceinline__ void cavity_cell(const Args& a, int i, int j, int k, const Post& post = Post{}) {
     using T = typename S::T;
     using R = typename S::Real;
@@ -224,7 +228,10 @@ __device__ __forceinline__ void cavity_cell(const Args& a, int i, int j, int k, 
         }
         sfor<Q>([&](auto A) {
             constexpr int al = decltype(A)::value;
-            const long long off = (long long)EX[al] + (long long)EY[al] * a.px + (long long)EZ[al] * a.pxy;
+            long long off = (long long)EX[al] + (long long)EY[al] * a.px + (long long)EZ[al] * a.pxy;
+            if constexpr (XSEL && EX[al] != 0) {
+                if ((unsigned)(i - EX[al]) >= (unsigned)a.nx) off -= EX[al];
+            }
             f[al] = S::ld(src + ((long long)al * a.ps + c - off), a.inv_scale);
         });
         if (i == 0 || i == a.nx - 1) {
