@@ -557,12 +557,49 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_links(const PArgs a) {
     }
 }
 
-// Every node: pull streaming (fluid.F90:75-161) with halfway bounce-back at
-// the box walls -- a pull out of a solid node reads the interpolated
-// bounce-back value k_links stored there -- then macro (fluid.F90:164-184;
-// rho every step for rhoAvg, u, v, w on the last step of a call) on fluid
-// nodes, and the next collision (fluid.F90:1-72).  Solid nodes run the same
-// arithmetic on values nobody reads.
+// The 19 populations node (i, j, k) pulls from the source buffer: pull
+// streaming (fluid.F90:75-161) with halfway bounce-back at the box walls -- a
+// pull out of a solid node reads the interpolated bounce-back value k_links
+// stored there.  The fluid kernel and the map update (for the rho of a node
+// it covers) take the same values through this one function.
+template <class S>
+__device__ __forceinline__ void pull_node(const PArgs& a, int i, int j, int k, typename S::Real (&f)[Q]) {
+    using T = typename S::T;
+    const int c = i + a.px * (j + a.ny * k);
+    const T* __restrict__ src = static_cast<const T*>(a.src);
+    sfor<Q>([&](auto B) {
+        constexpr int bd = decltype(B)::value;
+        constexpr int ex = EX[bd], ey = EY[bd], ez = EZ[bd];
+        if constexpr (bd == 0) {
+            f[0] = S::ld(src + c, a.inv_scale);
+        } else {
+            bool in = true;
+            if constexpr (ex != 0) in = in && (unsigned)(i - ex) < (unsigned)a.nx;
+            if constexpr (ey != 0) in = in && (unsigned)(j - ey) < (unsigned)a.ny;
+            if constexpr (ez != 0) in = in && (unsigned)(k - ez) < (unsigned)a.nz;
+            const long long off = (long long)ex + (long long)ey * a.px + (long long)ez * a.pxy;
+            f[bd] = in ? S::ld(src + (long long)bd * a.ps + (c - off), a.inv_scale)
+                       : S::ld(src + (long long)OPP[bd] * a.ps + c, a.inv_scale);
+        }
+    });
+}
+
+// rho of fluid node (i, j, k) in this step's streaming, as the fluid kernel
+// computes it (the same pulls and moments, so the same bits).
+template <class S>
+__device__ __forceinline__ double pulled_rho(const PArgs& a, int i, int j, int k) {
+    using R = typename S::Real;
+    R f[Q];
+    pull_node<S>(a, i, j, k, f);
+    const Moments<R, S::SHIFT> mo = moments<R, S::SHIFT>(f);
+    return S::SHIFT ? 1.0 + double(mo.drho) : double(mo.rho);
+}
+
+// Every node: the pulls (pull_node), then macro (fluid.F90:164-184) on fluid
+// nodes -- rho into the block's fluid sums every step (rhoAvg), rho, u, v, w
+// into the macro fields on the last step of a call -- and the next collision
+// (fluid.F90:1-72).  Solid nodes run the same arithmetic on values nobody
+// reads.
 template <class S, int COLL, bool MACRO>
 __global__ __launch_bounds__(256) void k_particle_step(const PArgs a) {
     using T = typename S::T;
@@ -575,30 +612,15 @@ __global__ __launch_bounds__(256) void k_particle_step(const PArgs a) {
     if (i < a.nx && j < a.ny) {
         const long long m = i + (long long)a.nx * (j + (long long)a.ny * k);
         const int c = i + a.px * (j + a.ny * k);
-        const T* __restrict__ src = static_cast<const T*>(a.src);
         R f[Q];
-        sfor<Q>([&](auto B) {
-            constexpr int bd = decltype(B)::value;
-            constexpr int ex = EX[bd], ey = EY[bd], ez = EZ[bd];
-            if constexpr (bd == 0) {
-                f[0] = S::ld(src + c, a.inv_scale);
-            } else {
-                bool in = true;
-                if constexpr (ex != 0) in = in && (unsigned)(i - ex) < (unsigned)a.nx;
-                if constexpr (ey != 0) in = in && (unsigned)(j - ey) < (unsigned)a.ny;
-                if constexpr (ez != 0) in = in && (unsigned)(k - ez) < (unsigned)a.nz;
-                const long long off = (long long)ex + (long long)ey * a.px + (long long)ez * a.pxy;
-                f[bd] = in ? S::ld(src + (long long)bd * a.ps + (c - off), a.inv_scale)
-                           : S::ld(src + (long long)OPP[bd] * a.ps + c, a.inv_scale);
-            }
-        });
+        pull_node<S>(a, i, j, k, f);
         const Moments<R, S::SHIFT> mo = moments<R, S::SHIFT>(f);
         if (a.owner[m] < 0) {
             const double rho = S::SHIFT ? 1.0 + double(mo.drho) : double(mo.rho);
-            a.rho[m] = rho;
             rho_fluid = rho;
             n_fluid = 1.0;
-            if constexpr (MACRO) {
+            if constexpr (MACRO) {  // other steps: rho goes into the block sums only
+                a.rho[m] = rho;
                 a.u[m] = double(mo.u);
                 a.v[m] = double(mo.v);
                 a.w[m] = double(mo.w);
@@ -786,8 +808,11 @@ __device__ __forceinline__ void for_sphere_nodes(const PArgs& a, const double* x
 
 // New map from the updated spheres; solids at rest with rho_s.  A newly
 // covered node (fluid in the old map) is handled by its first claimer only:
-// its rho (the fluid kernel's, this step) goes into the chain's correction,
-// then rho_s and u = 0 are written.
+// its rho of this step's streaming (pulled_rho: the fluid kernel's value,
+// recomputed from the same pulls, since the fluid kernel writes the rho field
+// on the last step of a call only) goes into the chain's correction, then
+// rho_s and u = 0 are written.
+template <class S>
 __global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner_out) {
     if (stopped(a)) return;
     const int id = blockIdx.x;
@@ -798,12 +823,12 @@ __global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner
     int ncov = 0;
     for_sphere_nodes(
         a, a.xc_new, id, [&](long long m) { return a.owner[m]; },
-        [&](int, int, int, long long m, int own) {
+        [&](int i, int j, int k, long long m, int own) {
             bool first = true;
             if (plain) owner_out[m] = id;
             else first = atomicMax(owner_out + m, id) < 0;
             if (own < 0 && first) {  // nodes solid before already hold rho_s, u = 0
-                cov += to_fix(a.rho[m]);
+                cov += to_fix(pulled_rho<S>(a, i, j, k));
                 ++ncov;
                 a.rho[m] = a.rho_solid;
                 a.u[m] = 0.0;
@@ -1173,7 +1198,7 @@ void particle_launches(lbm_ctx* c, hipStream_t st, bool macro) {
     hipLaunchKernelGGL(k_particle_update, dim3((np + UPDATE_WAVES - 1) / UPDATE_WAVES), dim3(64 * UPDATE_WAVES), 0, st,
                        a);
     hipLaunchKernelGGL(k_fill_int, dim3(2048), dim3(256), 0, st, s->owner[1 - s->cur], c->n, -1);
-    hipLaunchKernelGGL(k_particle_mark, dim3(np), dim3(256), 0, st, a, s->owner[1 - s->cur]);
+    hipLaunchKernelGGL(k_particle_mark<S>, dim3(np), dim3(256), 0, st, a, s->owner[1 - s->cur]);
     hipLaunchKernelGGL(k_refill_scan, dim3(scan_blocks(np)), dim3(RED_THREADS), 0, st, a);
     hipLaunchKernelGGL(k_fluid_chain, dim3(1), dim3(RED_THREADS), 0, st, a, scan_blocks(np), s->scal);
     a.step_off = 0;
