@@ -82,6 +82,12 @@ struct ParticleState {
     unsigned char* item_fluid1 = nullptr;  // per entry: the node x - e_a is in the box and fluid
     long long* slab_off = nullptr;  // np: first entry of each slab
     int* slab_cap = nullptr;        // np: entries of each slab
+    // link-list reuse: a list stays valid while the map near its sphere does
+    int* link_count = nullptr;  // np: entries of the last list built
+    int* link_lo = nullptr;     // 3 np: the bounding-box origin its entries are relative to
+    int* changed = nullptr;     // 2 np, by step parity: the map update changed nodes of the sphere
+    int* nbrs = nullptr;        // LINK_NBRS np: the spheres within LINK_GAP of each (k_particle_update)
+    int* nnbr = nullptr;        // np: their number (above LINK_NBRS: too many, always rebuild)
     int* refill = nullptr;        // refill_cap pairs (node, sphere) of uncovered nodes
     int* refill_count = nullptr;  // entries this step
     int refill_cap = 0;
@@ -108,6 +114,8 @@ constexpr int REFILL_BLOCKS = 128;  // k_refill_proc: lanes walk the list with a
 constexpr int GRAPH_STEPS = 10;     // steps per graph replay; even, so the buffers end where they started
 constexpr int SUM_BLOCKS = 256;      // k_refill_scan: at least this many blocks (slices of the fluid sums)
 constexpr int UPDATE_CHUNK = 1024;  // k_particle_update: spheres staged in LDS at a time (32 KiB)
+constexpr int LINK_NBRS = 16;     // neighbours a sphere's link list watches (more: rebuilt every step)
+constexpr double LINK_GAP = 4.5;  // surface gap within which another sphere can change the map near a list
 constexpr int UPDATE_WAVES = 4;     // k_particle_update: waves (spheres) per block, which share the staging
                                     // (16, staging a quarter as often on a quarter of the CUs: 18 -> 23 us)
 constexpr int FLUID_BLOCKS = 4096;  // k_fluid_sum: enough waves in flight to stream the map and rho
@@ -144,6 +152,12 @@ struct PArgs {
     unsigned char* item_fluid1;
     const long long* slab_off;
     const int* slab_cap;
+    int* link_count;
+    int* link_lo;
+    int* changed;  // [parity][np]
+    int* nbrs;
+    int* nnbr;
+    int par;       // the state's parity this step (which half of changed the map update sets)
     // sphere update
     double gravity, threshold_wall, stiff_wall, threshold_particle, stiff_particle;
     double *F, *T, *Ftot;
@@ -409,10 +423,26 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_links(const PArgs a) {
     __shared__ unsigned shell[SCAN_CHUNK];
     __shared__ int part[SCAN_CHUNK / SCAN_THREADS][SCAN_THREADS / 64];
     __shared__ int wave_tot[SCAN_THREADS / 64];
-    if (threadIdx.x == 0) halt = stopped(a);
+    __shared__ int rebuild;
+    const int id = blockIdx.x;
+    if (threadIdx.x < 64) {
+        // the list from the last step holds while the map near the sphere is
+        // unchanged: the sphere's own nodes and those of every sphere that
+        // comes within LINK_GAP of it (what a link, its fl1 flag and its
+        // order depend on); lane 0 the sphere, lanes 1.. its neighbours
+        const int* ch = a.changed + (long long)(1 - a.par) * a.np;
+        const int nn = a.nnbr[id];
+        bool c = false;
+        if (threadIdx.x == 0) c = ch[id] != 0 || nn > LINK_NBRS;
+        else if (threadIdx.x <= nn && nn <= LINK_NBRS) c = ch[a.nbrs[(long long)id * LINK_NBRS + threadIdx.x - 1]] != 0;
+        const bool any = __ballot(c) != 0ull;
+        if (threadIdx.x == 0) {
+            rebuild = any ? 1 : 0;
+            halt = stopped(a);
+        }
+    }
     __syncthreads();
     if (halt) return;
-    const int id = blockIdx.x;
     int lo[3], hi[3];
     sphere_box(a, a.xc, id, 1, lo, hi);
     const unsigned bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
@@ -430,7 +460,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_links(const PArgs a) {
         k = lo[2] + (int)(t / (bx * by));
     };
     int pos = 0;  // entries written so far (block-uniform)
-    for (unsigned base = 0; base < total; base += SCAN_CHUNK) {
+    if (!rebuild) {  // the last list, relative to the box it was built in
+        pos = a.link_count[id];
+        for (int d = 0; d < 3; ++d) lo[d] = a.link_lo[3 * id + d];
+    }
+    for (unsigned base = 0; rebuild && base < total; base += SCAN_CHUNK) {
         // 1. shell nodes of the chunk, compacted in box order: node
         //    base + u * 256 + tid goes after every (u' < u) and every earlier
         //    wave of the same u
@@ -520,6 +554,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_links(const PArgs a) {
     if (pos > cap) {  // cannot happen for a slab sized by the shell bound
         if (threadIdx.x == 0) stop(a, 5);
         return;
+    }
+    if (rebuild && threadIdx.x == 0) {
+        a.link_count[id] = pos;
+        for (int d = 0; d < 3; ++d) a.link_lo[3 * id + d] = lo[d];
     }
     // the entries written above are visible to every wave of the block (the
     // last barrier of step 1)
@@ -694,6 +732,7 @@ __global__ __launch_bounds__(64 * UPDATE_WAVES) void k_particle_update(const PAr
     const double scale_p = V * (a.rho_solid - rho_avg) * a.gravity / a.stiff_particle;
     const double px = a.xc[3 * pp], py = a.xc[3 * pp + 1], pz = a.xc[3 * pp + 2];
     bool close = false;  // another sphere within one node of this one
+    int nnb = 0;         // spheres within LINK_GAP of this one (wave-uniform)
     for (int q0 = 0; q0 < a.np; q0 += UPDATE_CHUNK) {
         const int len = min(UPDATE_CHUNK, a.np - q0);
         __syncthreads();  // the previous chunk is read
@@ -707,10 +746,22 @@ __global__ __launch_bounds__(64 * UPDATE_WAVES) void k_particle_update(const PAr
         // spheres would (chunks are whole multiples of 64)
         for (int t = lane; t < len; t += 64) {
             const int q = q0 + t;
-            if (q == p) continue;
             const double4 o = others[t];
             const double dx = px - o.x, dy = py - o.y, dz = pz - o.z;
             const double rs = rp + o.w;
+            // the neighbours whose map changes can touch this sphere's link
+            // list, in q order (every lane still active in this pass takes
+            // part, so nnb stays the same on them)
+            const double rl = rs + LINK_GAP;
+            const bool nb = q != p && fabs(dx) < rl && fabs(dy) < rl && fabs(dz) < rl && dist2(dx, dy, dz) < rl * rl;
+            const unsigned long long bb = __ballot(nb);
+            if (nb) {
+                const int at = nnb + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bb >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)bb, 0u));
+                if (at < LINK_NBRS) a.nbrs[(long long)p * LINK_NBRS + at] = q;
+            }
+            nnb += (int)__popcll(bb);
+            if (q == p) continue;
             const double r1 = rs + 1.0;
             if (fabs(dx) < r1 && fabs(dy) < r1 && fabs(dz) < r1 && dist2(dx, dy, dz) < r1 * r1) close = true;
             const double reach = rs + a.threshold_particle;
@@ -732,6 +783,8 @@ __global__ __launch_bounds__(64 * UPDATE_WAVES) void k_particle_update(const PAr
     for (int d = 0; d < 3; ++d) fs[d] = wave_sum(fs[d]);
     const bool alone = __ballot(close) == 0ull;
     if (lane != 0) return;
+    a.nnbr[p] = nnb;
+    a.changed[(long long)a.par * a.np + p] = 0;  // this step's map update sets it again
     // wall springs summed apart from the particle springs and added last, as
     // the reference keeps Fwxij apart from Fxij (particle_force.F90:192-193)
     const double scale_w = V * (a.rho_solid - a.rho0) * a.gravity / a.stiff_wall;
@@ -821,12 +874,18 @@ __global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner
     const bool plain = a.alone[id] != 0 && *(volatile const int*)(a.err + 3) == 0;
     unsigned long long cov = 0ull;  // this lane's newly covered nodes, then the wave's
     int ncov = 0;
+    int* chg = a.changed + (long long)a.par * a.np;  // spheres whose nodes this map update changes
+    bool moved = false;                               // a node of this sphere that was not its own
     for_sphere_nodes(
         a, a.xc_new, id, [&](long long m) { return a.owner[m]; },
         [&](int i, int j, int k, long long m, int own) {
             bool first = true;
             if (plain) owner_out[m] = id;
             else first = atomicMax(owner_out + m, id) < 0;
+            if (own != id) {
+                moved = true;
+                if (own >= 0) chg[own] = 1;  // a node another sphere held (rare: overlapping claims)
+            }
             if (own < 0 && first) {  // nodes solid before already hold rho_s, u = 0
                 cov += to_fix(pulled_rho<S>(a, i, j, k));
                 ++ncov;
@@ -836,6 +895,7 @@ __global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner
                 a.w[m] = 0.0;
             }
         });
+    if (__ballot(moved) != 0ull && (threadIdx.x & 63) == 0) chg[id] = 1;
     for (int off = 32; off > 0; off >>= 1) {
         cov += __shfl_down(cov, off, 64);
         ncov += __shfl_down(ncov, off, 64);
@@ -986,6 +1046,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_refill_scan(const PArgs a) {
             a, a.xc, id, [&](long long m) { return int2{a.owner[m], a.owner_new[m]}; },
             [&](int, int, int, long long m, int2 own) {
                 if (!(own.x == id && own.y < 0)) return;
+                a.changed[(long long)a.par * a.np + id] = 1;  // the sphere lost a node
                 const int slot = atomicAdd(a.refill_count, 1);
                 if (slot < a.refill_cap) {
                     a.refill[2 * slot] = (int)m;
@@ -1162,6 +1223,12 @@ PArgs particle_args(const lbm_ctx* c) {
     a.item_fluid1 = s->item_fluid1;
     a.slab_off = s->slab_off;
     a.slab_cap = s->slab_cap;
+    a.link_count = s->link_count;
+    a.link_lo = s->link_lo;
+    a.changed = s->changed;
+    a.nbrs = s->nbrs;
+    a.nnbr = s->nnbr;
+    a.par = s->cur;
     a.refill = s->refill;
     a.refill_count = s->refill_count;
     a.refill_cap = s->refill_cap;
@@ -1253,7 +1320,8 @@ void free_state(ParticleState* s) {
         if (s->owner[q]) (void)hipFree(s->owner[q]);
     }
     for (void* p : {(void*)s->r, (void*)s->F, (void*)s->T, (void*)s->Ftot, (void*)s->items, (void*)s->item_fluid1, (void*)s->slab_off,
-                    (void*)s->slab_cap, (void*)s->scal,
+                    (void*)s->slab_cap, (void*)s->link_count, (void*)s->link_lo, (void*)s->changed, (void*)s->nbrs,
+                    (void*)s->nnbr, (void*)s->scal,
                     (void*)s->red, (void*)s->err, (void*)s->alone, (void*)s->cpart, (void*)s->cpart2, (void*)s->chain, (void*)s->refill, (void*)s->refill_count})
         if (p) (void)hipFree(p);
 }
@@ -1360,6 +1428,14 @@ int lbm_particles_init(lbm_ctx* c, const lbm_particle_params* pp, const double* 
     if (e == hipSuccess) e = hipMalloc(&s->slab_cap, sizeof(int) * pp->np);
     if (e == hipSuccess) e = hipMemcpy(s->slab_off, off.data(), sizeof(long long) * pp->np, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->slab_cap, cap.data(), sizeof(int) * pp->np, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&s->link_count, sizeof(int) * pp->np);
+    if (e == hipSuccess) e = hipMalloc(&s->link_lo, sizeof(int) * 3 * pp->np);
+    if (e == hipSuccess) e = hipMalloc(&s->changed, sizeof(int) * 2 * pp->np);
+    if (e == hipSuccess) e = hipMalloc(&s->nbrs, sizeof(int) * LINK_NBRS * (size_t)pp->np);
+    if (e == hipSuccess) e = hipMalloc(&s->nnbr, sizeof(int) * pp->np);
+    // every list is built on the first step: all spheres changed, too many neighbours
+    if (e == hipSuccess) e = hipMemset(s->changed, 0x01, sizeof(int) * 2 * pp->np);
+    if (e == hipSuccess) e = hipMemset(s->nnbr, 0x7f, sizeof(int) * pp->np);
     if (e == hipSuccess) e = hipMalloc(&s->scal, sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&s->red, sizeof(double) * 2 * FLUID_BLOCKS);
     if (e == hipSuccess) e = hipMalloc(&s->err, 4 * sizeof(int));
