@@ -834,13 +834,20 @@ __global__ __launch_bounds__(64 * UPDATE_WAVES) void k_particle_update(const PAr
     if (!(moved < 0.0625)) atomicOr(a.err + 3, 1);
 }
 
+// Node (i, j, k) (0-based) is in the ball of sphere id with centres xc: the
+// walk's own test (the same expression, so the same answer for any node).
+__device__ __forceinline__ bool in_ball(const PArgs& a, const double* xc, int id, int i, int j, int k) {
+    const double cx = xc[3 * id], cy = xc[3 * id + 1], cz = xc[3 * id + 2], rr = a.r[id] * a.r[id];
+    return dist2(i + 1.0 - cx, j + 1.0 - cy, k + 1.0 - cz) <= rr;
+}
+
 // Nodes of sphere id (d^2 <= r^2, initial.F90:78-89), walked column by
 // column of its bounding box, four rows of a column at a time: load(m) for
 // the four nodes first, so their loads are in flight together, then
 // f(i, j, k, m, loaded value) for each.
 template <class L, class F>
 __device__ __forceinline__ void for_sphere_nodes(const PArgs& a, const double* xc, int id, L&& load, F&& f) {
-    using V = decltype(load(0LL));
+    using V = decltype(load(0, 0, 0, 0LL));
     int lo[3], hi[3];
     sphere_box(a, xc, id, 1, lo, hi);
     const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1;
@@ -859,7 +866,7 @@ __device__ __forceinline__ void for_sphere_nodes(const PArgs& a, const double* x
                 const int k = kb + u;
                 in[u] = k <= k1 && dist2(i + 1.0 - cx, j + 1.0 - cy, k + 1.0 - cz) <= rr;
                 m[u] = i + (long long)a.nx * (j + (long long)a.ny * k);
-                if (in[u]) v[u] = load(m[u]);
+                if (in[u]) v[u] = load(i, j, k, m[u]);
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u)
@@ -886,7 +893,10 @@ __global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner
     int* chg = a.changed + (long long)a.par * a.np;  // spheres whose nodes this map update changes
     bool moved = false;                               // a node of this sphere that was not its own
     for_sphere_nodes(
-        a, a.xc_new, id, [&](long long m) { return a.owner[m]; },
+        a, a.xc_new, id,
+        // the old owner; a sphere with no other within a node held every
+        // node of its old ball alone, so those need no load
+        [&](int i, int j, int k, long long m) { return plain && in_ball(a, a.xc, id, i, j, k) ? id : a.owner[m]; },
         [&](int i, int j, int k, long long m, int own) {
             bool first = true;
             if (plain) owner_out[m] = id;
@@ -919,7 +929,8 @@ __global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner
 __global__ __launch_bounds__(256) void k_particle_init_mark(const PArgs a, int* owner_out) {
     const int id = blockIdx.x;
     for_sphere_nodes(
-        a, a.xc, id, [](long long) { return 0; }, [&](int, int, int, long long m, int) { atomicMax(owner_out + m, id); });
+        a, a.xc, id, [](int, int, int, long long) { return 0; },
+        [&](int, int, int, long long m, int) { atomicMax(owner_out + m, id); });
 }
 
 // rho = rho_s in solids and rho0 in the fluid, u = 0, and the first
@@ -1052,7 +1063,12 @@ __global__ __launch_bounds__(RED_THREADS) void k_refill_scan(const PArgs a) {
     const int id = blockIdx.x;
     if (id < a.np)
         for_sphere_nodes(
-            a, a.xc, id, [&](long long m) { return int2{a.owner[m], a.owner_new[m]}; },
+            a, a.xc, id,
+            // a node of the new ball was claimed by the map update: not
+            // uncovered, no load (only the trailing crescent is checked)
+            [&](int i, int j, int k, long long m) {
+                return in_ball(a, a.xc_new, id, i, j, k) ? int2{id, id} : int2{a.owner[m], a.owner_new[m]};
+            },
             [&](int, int, int, long long m, int2 own) {
                 if (!(own.x == id && own.y < 0)) return;
                 a.changed[(long long)a.par * a.np + id] = 1;  // the sphere lost a node
