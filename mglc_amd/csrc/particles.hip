@@ -116,7 +116,10 @@ constexpr int GRAPH_STEPS = 10;     // steps per graph replay; even, so the buff
 constexpr int SUM_BLOCKS = 256;      // k_refill_scan: at least this many blocks (slices of the fluid sums)
 constexpr int UPDATE_CHUNK = 1024;  // k_particle_update: spheres staged in LDS at a time (32 KiB)
 constexpr int LINK_NBRS = 16;     // neighbours a sphere's link list watches (more: rebuilt every step)
-constexpr double LINK_GAP = 4.5;  // surface gap within which another sphere can change the map near a list
+// surface gap within which another sphere can change the map near a list: a
+// list reads owners up to r + 2.92 from its centre (x - e_a of a shell node),
+// plus 1/4 node of motion for each of the two spheres, with room to spare
+constexpr double LINK_GAP = 4.5;
 constexpr int UPDATE_WAVES = 4;     // k_particle_update: waves (spheres) per block, which share the staging
                                     // (16, staging a quarter as often on a quarter of the CUs: 18 -> 23 us)
 constexpr int FLUID_BLOCKS = 4096;  // k_fluid_sum: enough waves in flight to stream the map and rho
@@ -914,7 +917,11 @@ __global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner
                 a.w[m] = 0.0;
             }
         });
-    if (__ballot(moved) != 0ull && (threadIdx.x & 63) == 0) chg[id] = 1;
+    // the neighbour lists' margin holds for spheres that move less than 1/4
+    // node per step; if one moved more (k_particle_update's flag), every list
+    // is built afresh
+    const bool fast = *(volatile const int*)(a.err + 3) != 0;
+    if ((fast || __ballot(moved) != 0ull) && (threadIdx.x & 63) == 0) chg[id] = 1;
     for (int off = 32; off > 0; off >>= 1) {
         cov += __shfl_down(cov, off, 64);
         ncov += __shfl_down(ncov, off, 64);
