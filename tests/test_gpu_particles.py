@@ -108,6 +108,31 @@ def test_fp64_moving_spheres_match_oracle(oracle_lib):
         assert np.abs(got[name] - ref[name]).max() <= 1e-10 * link_scale * r.max(), name
 
 
+def test_fast_spheres_match_oracle(oracle_lib):
+    """Two spheres launched at 0.30 and 0.26 nodes per step: a sphere that
+    moves a quarter node or more in a step makes the map update rebuild every
+    sphere's link list for the next step (the lists are otherwise kept while
+    the map near their sphere is unchanged).  24 steps: the first eager, a
+    graph replay of ten, the rest direct."""
+    O = oracle_lib
+    n, xc, r = THREE["n"], THREE["xc"], THREE["r"]
+    U = np.array([[0.30, 0.0, 0.0], [0.0, 0.0, 0.0], [-0.26, 0.0, 0.0]])
+    steps, g = 24, 1e-4
+    got = gpu_particles(box(*n), xc, r, steps, U=U, gravity=g)
+    ref = O.particle_run(O.particle_params(*n, 3, 0.8, gravity=g), steps, xc, r, U=U)
+    assert ref["rc"] == 0
+    assert np.abs(got["xc"][0, 0] - xc[0, 0]) > 2.5  # it did move fast
+    assert np.array_equal(got["owner"], ref["owner"])
+    assert np.array_equal(got["owner"], brute_map(got["owner"].shape, got["xc"], r))
+    for name in ("rho", "u", "v", "w"):
+        assert relmax(got[name], ref[name]) <= FIELD_TOL, name
+    for name in ("xc", "U", "Om"):
+        assert relmax(got[name], ref[name]) <= 1e-12, name
+    link_scale = 1.0 / 18.0
+    for name in ("F", "T"):
+        assert np.abs(got[name] - ref[name]).max() <= 1e-10 * link_scale * r.max(), name
+
+
 def test_fp64_jittered_grid_with_springs_matches_oracle(oracle_lib):
     """Eight spheres on the reference's jittered grid layout
     (mpi_particle/initial.F90:48-84), packed so that particle-particle gaps
