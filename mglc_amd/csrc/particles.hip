@@ -86,7 +86,8 @@ struct ParticleState {
     // link-list reuse: a list stays valid while the map near its sphere does
     int* link_count = nullptr;  // np: entries of the last list built
     int* link_lo = nullptr;     // 3 np: the bounding-box origin its entries are relative to
-    int* changed = nullptr;     // 2 np, by step parity: the map update changed nodes of the sphere
+    int* changed = nullptr;     // 2 np + 2, by step parity: the map update changed a node the sphere's list
+                                //  can see; the last two words: every list is rebuilt
     int* nbrs = nullptr;        // LINK_NBRS np: the spheres within LINK_GAP of each (k_particle_update)
     int* nnbr = nullptr;        // np: their number (above LINK_NBRS: too many, always rebuild)
     int* refill = nullptr;        // refill_cap pairs (node, sphere) of uncovered nodes
@@ -158,7 +159,7 @@ struct PArgs {
     const int* slab_cap;
     int* link_count;
     int* link_lo;
-    int* changed;  // [parity][np]
+    int* changed;  // [parity][np], then [parity] rebuild every list
     int* nbrs;
     int* nnbr;
     int par;       // the state's parity this step (which half of changed the map update sets)
@@ -412,13 +413,11 @@ __device__ __forceinline__ void link_end(const PArgs& a, const SphereNow& sp, co
 //     one pass at a block-wide running offset: per round of 256 shell nodes
 //     the waves in order, inside a wave direction by direction, lane by lane;
 //     A link is (x fluid, x + e_a owned by the sphere), a function of the map
-//     alone, so step 1 runs only when the map near the sphere changed in the
-//     last map update: its own nodes, or those of a sphere within LINK_GAP of
-//     it (k_particle_mark and k_refill_scan flag the spheres whose nodes
-//     change, k_particle_update lists the neighbours).  Otherwise the last
-//     list is used as it stands, with the box origin it was built in -- the
-//     same links, in the order of that build.  At 192^3 with 2000 spheres this
-//     saves most of the scan: 348 -> 304 us;
+//     alone, so step 1 runs only when the last map update changed the owner
+//     of a node the list can see (k_particle_mark and k_refill_scan flag the
+//     spheres within reach of each node they change, flag_near).  Otherwise
+//     the last list is used as it stands, with the box origin it was built in
+//     -- the same links, in the order of that build;
 //  2. one lane per link, two at a time (entries e and e + 256, e = 64 w +
 //     512 t + lane for wave w): link_begin of both, then link_end of both;
 //  3. force and torque: per lane in entry order, per wave by a fixed shuffle
@@ -437,21 +436,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_links(const PArgs a) {
     __shared__ int wave_tot[SCAN_THREADS / 64];
     __shared__ int rebuild;
     const int id = blockIdx.x;
-    if (threadIdx.x < 64) {
-        // the list from the last step holds while the map near the sphere is
-        // unchanged: the sphere's own nodes and those of every sphere that
-        // comes within LINK_GAP of it (what a link, its fl1 flag and its
-        // order depend on); lane 0 the sphere, lanes 1.. its neighbours
-        const int* ch = a.changed + (long long)(1 - a.par) * a.np;
-        const int nn = a.nnbr[id];
-        bool c = false;
-        if (threadIdx.x == 0) c = ch[id] != 0 || nn > LINK_NBRS;
-        else if (threadIdx.x <= nn && nn <= LINK_NBRS) c = ch[a.nbrs[(long long)id * LINK_NBRS + threadIdx.x - 1]] != 0;
-        const bool any = __ballot(c) != 0ull;
-        if (threadIdx.x == 0) {
-            rebuild = any ? 1 : 0;
-            halt = stopped(a);
-        }
+    if (threadIdx.x == 0) {
+        // the list from the last step holds unless the last map update
+        // changed the owner of a node the list can see (flag_near), or asked
+        // for every list to be rebuilt
+        rebuild = (a.changed[(long long)(1 - a.par) * a.np + id] != 0 ||
+                   a.changed[2LL * a.np + (1 - a.par)] != 0) ? 1 : 0;
+        halt = stopped(a);
     }
     __syncthreads();
     if (halt) return;
@@ -797,6 +788,7 @@ __global__ __launch_bounds__(64 * UPDATE_WAVES) void k_particle_update(const PAr
     if (lane != 0) return;
     a.nnbr[p] = nnb;
     a.changed[(long long)a.par * a.np + p] = 0;  // this step's map update sets it again
+    if (p == 0) a.changed[2LL * a.np + a.par] = 0;
     // wall springs summed apart from the particle springs and added last, as
     // the reference keeps Fwxij apart from Fxij (particle_force.F90:192-193)
     const double scale_w = V * (a.rho_solid - a.rho0) * a.gravity / a.stiff_wall;
@@ -842,6 +834,31 @@ __global__ __launch_bounds__(64 * UPDATE_WAVES) void k_particle_update(const PAr
 __device__ __forceinline__ bool in_ball(const PArgs& a, const double* xc, int id, int i, int j, int k) {
     const double cx = xc[3 * id], cy = xc[3 * id + 1], cz = xc[3 * id + 2], rr = a.r[id] * a.r[id];
     return dist2(i + 1.0 - cx, j + 1.0 - cy, k + 1.0 - cz) <= rr;
+}
+
+// The map update changes the owner of node (i, j, k) (0-based), which sphere t
+// claims or loses: flag the spheres whose link lists can see the node for a
+// rebuild next step -- t and those of t's neighbours (k_particle_update's
+// list) whose new ball comes within LIST_REACH of it.  A list reads owners up
+// to 2.92 nodes beyond its sphere's nodes (x - e_a of a shell node), so no
+// other sphere can see the node: one that did would come within
+// r_t + r + 3.25 of t, inside the neighbours' margin.  If t's neighbour list
+// overflowed, every list is rebuilt.
+constexpr double LIST_REACH = 3.0;
+__device__ __forceinline__ void flag_near(const PArgs& a, int t, int i, int j, int k) {
+    int* chg = a.changed + (long long)a.par * a.np;
+    chg[t] = 1;
+    const int nn = a.nnbr[t];
+    if (nn > LINK_NBRS) {
+        a.changed[2LL * a.np + a.par] = 1;
+        return;
+    }
+    for (int q = 0; q < nn; ++q) {
+        const int u = a.nbrs[(long long)t * LINK_NBRS + q];
+        const double R = a.r[u] + LIST_REACH;
+        if (dist2(i + 1.0 - a.xc_new[3 * u], j + 1.0 - a.xc_new[3 * u + 1], k + 1.0 - a.xc_new[3 * u + 2]) <= R * R)
+            chg[u] = 1;
+    }
 }
 
 // Nodes of sphere id (d^2 <= r^2, initial.F90:78-89), walked column by
@@ -893,8 +910,7 @@ __global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner
     const bool plain = a.alone[id] != 0 && *(volatile const int*)(a.err + 3) == 0;
     unsigned long long cov = 0ull;  // this lane's newly covered nodes, then the wave's
     int ncov = 0;
-    int* chg = a.changed + (long long)a.par * a.np;  // spheres whose nodes this map update changes
-    bool moved = false;                               // a node of this sphere that was not its own
+    int* chg = a.changed + (long long)a.par * a.np;  // spheres whose lists see a node this update changes
     for_sphere_nodes(
         a, a.xc_new, id,
         // the old owner; a sphere with no other within a node held every
@@ -904,8 +920,8 @@ __global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner
             bool first = true;
             if (plain) owner_out[m] = id;
             else first = atomicMax(owner_out + m, id) < 0;
-            if (own != id) {
-                moved = true;
+            if (own != id) {  // the node changes owner
+                flag_near(a, id, i, j, k);
                 if (own >= 0) chg[own] = 1;  // a node another sphere held (rare: overlapping claims)
             }
             if (own < 0 && first) {  // nodes solid before already hold rho_s, u = 0
@@ -920,8 +936,7 @@ __global__ __launch_bounds__(256) void k_particle_mark(const PArgs a, int* owner
     // the neighbour lists' margin holds for spheres that move less than 1/4
     // node per step; if one moved more (k_particle_update's flag), every list
     // is built afresh
-    const bool fast = *(volatile const int*)(a.err + 3) != 0;
-    if ((fast || __ballot(moved) != 0ull) && (threadIdx.x & 63) == 0) chg[id] = 1;
+    if (threadIdx.x == 0 && *(volatile const int*)(a.err + 3) != 0) a.changed[2LL * a.np + a.par] = 1;
     for (int off = 32; off > 0; off >>= 1) {
         cov += __shfl_down(cov, off, 64);
         ncov += __shfl_down(ncov, off, 64);
@@ -1076,9 +1091,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_refill_scan(const PArgs a) {
             [&](int i, int j, int k, long long m) {
                 return in_ball(a, a.xc_new, id, i, j, k) ? int2{id, id} : int2{a.owner[m], a.owner_new[m]};
             },
-            [&](int, int, int, long long m, int2 own) {
+            [&](int i, int j, int k, long long m, int2 own) {
                 if (!(own.x == id && own.y < 0)) return;
-                a.changed[(long long)a.par * a.np + id] = 1;  // the sphere lost a node
+                flag_near(a, id, i, j, k);  // the sphere lost the node
                 const int slot = atomicAdd(a.refill_count, 1);
                 if (slot < a.refill_cap) {
                     a.refill[2 * slot] = (int)m;
@@ -1462,11 +1477,11 @@ int lbm_particles_init(lbm_ctx* c, const lbm_particle_params* pp, const double* 
     if (e == hipSuccess) e = hipMemcpy(s->slab_cap, cap.data(), sizeof(int) * pp->np, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&s->link_count, sizeof(int) * pp->np);
     if (e == hipSuccess) e = hipMalloc(&s->link_lo, sizeof(int) * 3 * pp->np);
-    if (e == hipSuccess) e = hipMalloc(&s->changed, sizeof(int) * 2 * pp->np);
+    if (e == hipSuccess) e = hipMalloc(&s->changed, sizeof(int) * (2 * (size_t)pp->np + 2));
     if (e == hipSuccess) e = hipMalloc(&s->nbrs, sizeof(int) * LINK_NBRS * (size_t)pp->np);
     if (e == hipSuccess) e = hipMalloc(&s->nnbr, sizeof(int) * pp->np);
     // every list is built on the first step: all spheres changed, too many neighbours
-    if (e == hipSuccess) e = hipMemset(s->changed, 0x01, sizeof(int) * 2 * pp->np);
+    if (e == hipSuccess) e = hipMemset(s->changed, 0x01, sizeof(int) * (2 * (size_t)pp->np + 2));
     if (e == hipSuccess) e = hipMemset(s->nnbr, 0x7f, sizeof(int) * pp->np);
     if (e == hipSuccess) e = hipMalloc(&s->scal, sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&s->red, sizeof(double) * 2 * FLUID_BLOCKS);
