@@ -5,24 +5,28 @@
 //   k_links           one block per sphere: the boundary links (fluid node x,
 //                     direction a with x + e_a inside the sphere) of the
 //                     sphere's shell, compacted in a fixed order into the
-//                     sphere's slab (or the last step's list, while the map
-//                     near the sphere is unchanged); then one lane per link: wall distance q
-//                     by bisection, quadratic interpolated bounce-back with the
-//                     moving-wall term (particle_bounceback.F90:37-141), the
-//                     value stored in the solid node's own slot of the source
-//                     buffer (the slot the fluid node pulls from), and the
-//                     link's momentum exchange and torque
-//                     (particle_force.F90:31-92), summed in FP64 in a fixed
-//                     order: force and torque are deterministic
+//                     sphere's slab (or the last list, while no node it can
+//                     see changed owner); then one lane per link: wall
+//                     distance q by bisection, quadratic interpolated
+//                     bounce-back with the moving-wall term
+//                     (particle_bounceback.F90:37-141), the value stored in
+//                     the solid node's own slot of the source buffer (the
+//                     slot the fluid node pulls from), and the link's
+//                     momentum exchange and torque (particle_force.F90:
+//                     31-92), summed in FP64 in a fixed order: force and
+//                     torque are deterministic
 //   k_particle_step   every node: plain pull streaming with halfway bounce-back
-//                     at the box walls, macro (fluid nodes), collision, and
-//                     per block the sums of rho and count over fluid nodes
+//                     at the box walls, collision, per block the sums of rho
+//                     and count over fluid nodes; the macro fields on the last
+//                     step of a call
 //   k_particle_update one wave per sphere: spring and buoyancy forces and the
 //                     kinematics (particle_force.F90:96-191,
 //                     particle_update.F90:26-42); stop codes for
-//                     interpenetration and wall penetration
+//                     interpenetration and wall penetration; the neighbours
+//                     within LINK_GAP
 //   k_particle_mark   new solid map, rho_s and u = 0 in solids
-//                     (particle_update.F90:69-91)
+//                     (particle_update.F90:69-91); flags the link lists that
+//                     see a node changing owner
 //   k_refill_scan     one block per sphere: its uncovered nodes (old solid,
 //                     new fluid) appended to a list
 //   k_fluid_chain     rhoAvg over the new fluid nodes (particle_bounceback.F90:
