@@ -738,6 +738,16 @@ __global__ __launch_bounds__(64 * UPDATE_WAVES) void k_particle_update(const PAr
     double fs[3] = {0.0, 0.0, 0.0};
     const double scale_p = V * (a.rho_solid - rho_avg) * a.gravity / a.stiff_particle;
     const double px = a.xc[3 * pp], py = a.xc[3 * pp + 1], pz = a.xc[3 * pp + 2];
+    // the sphere's own state for lane 0's update after the pair loop, loaded
+    // now so that its latency runs under the loop
+    double Fl[3], Tl[3], Ul[3], Oml[3];
+    for (int d = 0; d < 3; ++d) {
+        Fl[d] = a.F[3 * pp + d];
+        Tl[d] = a.T[3 * pp + d];
+        Ul[d] = a.U[3 * pp + d];
+        Oml[d] = a.Om[3 * pp + d];
+    }
+    const double pc[3] = {px, py, pz};
     bool close = false;  // another sphere within one node of this one
     int nnb = 0;         // spheres within LINK_GAP of this one (wave-uniform)
     for (int q0 = 0; q0 < a.np; q0 += UPDATE_CHUNK) {
@@ -799,8 +809,8 @@ __global__ __launch_bounds__(64 * UPDATE_WAVES) void k_particle_update(const PAr
     const double N[3] = {(double)a.nx, (double)a.ny, (double)a.nz};
     double fw[3] = {0.0, 0.0, 0.0};
     for (int d = 0; d < 3; ++d) {
-        const double lo = a.xc[3 * p + d] - rp - 1.0;
-        const double hi = N[d] - a.xc[3 * p + d] - rp;
+        const double lo = pc[d] - rp - 1.0;
+        const double hi = N[d] - pc[d] - rp;
         if (lo < 0.0 || hi < 0.0) stop(a, 2);
         if (lo < a.threshold_wall) {
             const double g = (lo - a.threshold_wall) / a.threshold_wall;
@@ -811,19 +821,19 @@ __global__ __launch_bounds__(64 * UPDATE_WAVES) void k_particle_update(const PAr
             fw[d] -= scale_w * (g * g);
         }
     }
-    double Fn[3] = {a.F[3 * p] + fs[0] + fw[0], a.F[3 * p + 1] + fs[1] + fw[1],
-                    a.F[3 * p + 2] - (a.rho_solid - rho_avg) * V * a.gravity + fs[2] + fw[2]};
+    double Fn[3] = {Fl[0] + fs[0] + fw[0], Fl[1] + fs[1] + fw[1],
+                    Fl[2] - (a.rho_solid - rho_avg) * V * a.gravity + fs[2] + fw[2]};
     const double mass = a.rho_solid * V, inertia = 0.4 * mass * rp * rp;
     double moved = 0.0;
     for (int d = 0; d < 3; ++d) {
         const double acc = Fn[d] / mass;
-        const double Uold = a.U[3 * p + d];
+        const double Uold = Ul[d];
         a.U_out[3 * p + d] = Uold + acc;
-        const double x = a.xc[3 * p + d] + Uold + 0.5 * acc;
+        const double x = pc[d] + Uold + 0.5 * acc;
         a.xc_out[3 * p + d] = x;
-        a.Om_out[3 * p + d] = a.Om[3 * p + d] + a.T[3 * p + d] / inertia;
+        a.Om_out[3 * p + d] = Oml[d] + Tl[d] / inertia;
         a.Ftot[3 * p + d] = Fn[d];
-        moved += (x - a.xc[3 * p + d]) * (x - a.xc[3 * p + d]);
+        moved += (x - pc[d]) * (x - pc[d]);
     }
     // k_particle_mark may claim this sphere's new nodes with plain stores if
     // no other sphere was within one node of it and no sphere moves 1/4 node
